@@ -1,0 +1,199 @@
+#!/usr/bin/env python3
+"""Headline benchmark of the MI355X FMM-FFT.
+
+Workload (the precision form the metric is quoted on): one complex64 FFT of
+N = 2^26 points with P = 64 blocks (M = 2^20), Chebyshev order 16 and 64
+points per leaf box; input = 16 tones at seeded random bins with amplitudes
+10^U(-4,0) plus complex N(0, 1e-3) noise, generated on the device and
+resident in HBM before timing.  A step is one whole transform (plan built
+once, as the reference's mfti/mft timing loop does, src/fmmfft.c:1249-1275).
+
+Reported: GFLOP/s = 5 N log2 N / t of the complex64 path, the complex128
+("plain FP64") path at the same N for the ratio, the rel-L2 distance between
+the two outputs, p50/p99 step latency, a per-stage breakdown from HIP events
+recorded on the execute stream, the roofline of the dominant kernel, and the
+CPU baseline (the oracle's FMM-FFT restatement timed on the host cores on a
+bounded sample of the same workload).
+"""
+import argparse
+import importlib.util
+import json
+import math
+import os
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+
+N, P_BLOCKS, TERMS, LEAF = 1 << 26, 64, 16, 64
+PEAK_FP32_TFLOPS = 157.3   # MI355X dense FP32 matrix
+PEAK_FP64_TFLOPS = 78.6
+PEAK_HBM_GBS = 8000.0
+
+
+def load_pkg():
+    if "fmmfft_amd" in sys.modules:
+        return sys.modules["fmmfft_amd"]
+    spec = importlib.util.spec_from_file_location("fmmfft_amd", os.path.join(REPO, "fmm-fft_amd", "__init__.py"))
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules["fmmfft_amd"] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def tones_on_device(torch, n, seed):
+    g = torch.Generator(device="cuda").manual_seed(seed)
+    x = torch.empty(n, dtype=torch.complex128, device="cuda")
+    x.real.normal_(0.0, 1e-3, generator=g)
+    x.imag.normal_(0.0, 1e-3, generator=g)
+    k = torch.arange(n, dtype=torch.int64, device="cuda")
+    gc = torch.Generator().manual_seed(seed)
+    for _ in range(16):
+        f = int(torch.randint(0, n, (1,), generator=gc))
+        amp = 10 ** float(torch.empty(1).uniform_(-4, 0, generator=gc))
+        ph = float(torch.empty(1).uniform_(0, 2 * math.pi, generator=gc))
+        ang = (2 * math.pi / n) * ((f * k) % n).to(torch.float64) + ph
+        x += amp * torch.polar(torch.ones_like(ang), ang)
+    return x
+
+
+def time_plan(torch, plan, x, out, steps, warmup, profile):
+    for _ in range(warmup):
+        plan.execute(x, out)
+    torch.cuda.synchronize()
+    if profile:
+        plan.profile_begin(steps)
+    st = torch.cuda.current_stream()
+    evs = [torch.cuda.Event(enable_timing=True) for _ in range(steps + 1)]
+    torch.cuda.synchronize()
+    evs[0].record(st)
+    for i in range(steps):
+        plan.execute(x, out)
+        evs[i + 1].record(st)
+    torch.cuda.synchronize()
+    lat = [evs[i].elapsed_time(evs[i + 1]) for i in range(steps)]
+    total = evs[0].elapsed_time(evs[-1])
+    stages = plan.profile_end() if profile else None
+    return total / steps, lat, stages
+
+
+def pct(v, q):
+    v = sorted(v)
+    return v[min(len(v) - 1, int(math.ceil(q / 100 * len(v))) - 1)]
+
+
+def cpu_baseline(sample_log2n):
+    """The oracle's FMM-FFT (C restatement of src/fmmfft.c, OpenMP over the
+    host cores) on N = 2^sample_log2n with the workload's P, order and leaf;
+    plan once, 3 runs, first discarded, median of the rest."""
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import oracle  # checker / CPU baseline only
+
+    n = 1 << sample_log2n
+    pl = oracle.OraclePlan(n, P_BLOCKS, TERMS, LEAF)
+    x = oracle.normal(n, 1)
+    ts = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        pl.fmmfft(x)
+        ts.append(time.perf_counter() - t0)
+    t = sorted(ts[1:])[len(ts[1:]) // 2]
+    return {
+        "value": round(5 * n * math.log2(n) / t / 1e9, 3),
+        "unit": "GFLOP/s",
+        "cores": oracle.num_threads(),
+        "kind": "port",
+        "sample": f"oracle FMM-FFT (C restatement of src/fmmfft.c), N=2^{sample_log2n} complex128, "
+                  f"P={P_BLOCKS}, p={TERMS}, leaf {LEAF}, median of 2 timed runs after 1 warm-up, "
+                  f"{t:.3f} s per transform",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-fp64", action="store_true", help="skip the complex128 comparison run")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-sample-log2n", type=int, default=23)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this FMM-FFT runs on one GPU; --gpus {args.gpus} is not supported "
+              "(multi-GPU sharding is design only)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    fmm = load_pkg()
+    torch.cuda.set_device(0)
+    plan32 = fmm.Plan(N, P_BLOCKS, TERMS, LEAF, fmm.C64)
+    x64 = tones_on_device(torch, N, 1234)
+    x32 = x64.to(torch.complex64)
+    out32 = torch.empty_like(x32)
+    ms32, lat32, stages = time_plan(torch, plan32, x32, out32, args.steps, args.warmup, True)
+    gflops32 = 5 * N * math.log2(N) / (ms32 * 1e-3) / 1e9
+
+    extra = {}
+    if not args.no_fp64:
+        del x64
+        plan64 = fmm.Plan(N, P_BLOCKS, TERMS, LEAF, fmm.C128)
+        xin64 = x32.to(torch.complex128)
+        out64 = torch.empty_like(xin64)
+        steps64 = max(1, min(args.steps, 10))
+        ms64, lat64, stages64 = time_plan(torch, plan64, xin64, out64, steps64, 2, True)
+        gflops64 = 5 * N * math.log2(N) / (ms64 * 1e-3) / 1e9
+        d = out32.to(torch.complex128) - out64
+        rel = float(torch.linalg.vector_norm(d) / torch.linalg.vector_norm(out64))
+        extra.update({
+            "fp64_plain_form": {"value": round(gflops64, 1), "unit": "GFLOP/s", "ms_per_step": round(ms64, 4),
+                                "stage_ms": {k: round(v / steps64, 4) for k, v in stages64[0].items()}},
+            "speedup_fp32_vs_fp64": round(gflops32 / gflops64, 3),
+            "rel_l2_fp32_vs_fp64": rel,
+        })
+        del plan64, xin64, out64, d
+
+    per_stage = {k: v / stages[1] for k, v in stages[0].items()}
+    dom = max(per_stage, key=per_stage.get)
+    M = N // P_BLOCKS
+    S = P_BLOCKS - 1
+    if dom == "leaf":
+        work = 4.0 * S * M * (3 * LEAF + TERMS)  # complex x real MACs of near field + L2T
+        achieved = work / (per_stage[dom] * 1e-3) / 1e12
+        roof = {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": None, "kernel": "k_leaf<float>",
+                "algorithmic": f"{work:.4g} flop per launch"}
+    else:
+        by = 2.0 * N * 8  # one read + one write of the complex64 signal
+        achieved = by / (per_stage[dom] * 1e-3) / 1e9
+        roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None, "kernel": dom,
+                "algorithmic": f"{by:.4g} bytes per launch"}
+
+    line = {
+        "metric": "GFLOP/s (5N·log2N/t) at N=2^26 FP32-MFMA vs FP64 at rel-L2≤1e-6; p99 latency",
+        "value": round(gflops32, 1),
+        "unit": "GFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms32, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (16 tones + complex N(0,1e-3) noise, generated on the device)",
+        "config": {"workload": "FMM-FFT precision form: N=2^26 complex64, P=64 (M=2^20), p=16, leaf 64",
+                   "n": N, "p_blocks": P_BLOCKS, "terms": TERMS, "leaf": LEAF, "batch": 1},
+        "latency_ms": {"p50": round(pct(lat32, 50), 4), "p99": round(pct(lat32, 99), 4)},
+        "stage_ms": {k: round(v, 4) for k, v in per_stage.items()},
+        "roofline": roof,
+    }
+    line.update(extra)
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline(args.cpu_sample_log2n)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

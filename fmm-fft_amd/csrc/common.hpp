@@ -1,0 +1,77 @@
+// Shared device types and helpers of the MI355X FMM-FFT kernels (gfx950).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace fmmfft {
+
+// Complex value in the plan's storage precision; 8- or 16-byte aligned so
+// that a load is one dwordx2 / dwordx4.
+template <typename T>
+struct alignas(2 * sizeof(T)) cx {
+    T re, im;
+};
+
+using c32 = cx<float>;
+using c64 = cx<double>;
+
+// Slots of the per-level translation operators (the reference's shftfn,
+// shftfp, flip2n, flip2p, flip3n, flip3p, shftln, shftlp,
+// src/fmmfft.c:281-290).  Level slot L holds the M2M pair with a child at
+// level L+3, the flips of level L+3 and the L2L pair into a child at level
+// L+3.
+enum : int {
+    OP_SHFTFN = 0,
+    OP_SHFTFP = 1,
+    OP_FLIP2N = 2,
+    OP_FLIP2P = 3,
+    OP_FLIP3N = 4,
+    OP_FLIP3P = 5,
+    OP_SHFTLN = 6,
+    OP_SHFTLP = 7,
+    NOPS = 8
+};
+
+// Operator tables in one precision, all device pointers.
+template <typename T>
+struct Tables {
+    const T *initch;  // [b][terms]
+    const T *ops;     // [n-2][8][terms][terms]
+    const T *topflp;  // [terms][terms]
+    const T *evalm;   // [P-1][b][terms]
+    const T *evalmh;  // [P/2][terms]
+    const T *kv;      // [P-1][4b]   K_sc(d), d = -2b .. 2b-1
+};
+
+// Geometry of one transform; passed by value to every kernel.
+struct Geom {
+    int64_t N, M;   // N = P * M
+    int P, S;       // S = P - 1 sections carry a cot matvec
+    int terms, b;   // Chebyshev order, points per leaf box
+    int s, n;       // leaf boxes per section (2^n), tree depth
+};
+
+// Complex offset of level l (levels 2..n, 2^l nodes of [S][2][terms] reals).
+__host__ __device__ inline int64_t level_off(const Geom &g, int l) {
+    return ((int64_t(1) << l) - 4) * int64_t(g.S) * 2 * g.terms;
+}
+__host__ __device__ inline int64_t tree_len(const Geom &g) {
+    return ((int64_t(1) << (g.n + 1)) - 4) * int64_t(g.S) * 2 * g.terms;
+}
+
+__device__ inline int pmod(int a, int m) {
+    int r = a % m;
+    return r < 0 ? r + m : r;
+}
+
+__device__ inline c64 cmul(c64 a, c64 b) {
+    return c64{a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+template <typename T>
+__device__ inline cx<T> cmulT(cx<T> a, cx<T> b) {
+    return cx<T>{a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+
+}  // namespace fmmfft

@@ -1,0 +1,610 @@
+// C ABI of the MI355X FMM-FFT: plans, device tables, workspace and the
+// execute launch sequence (include/fmmfft.h).
+#include "../../include/fmmfft.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cstring>
+#include <new>
+
+#include "common.hpp"
+#include "kernels_fft.hpp"
+#include "kernels_fmm.hpp"
+#include "kernels_ops.hpp"
+
+using namespace fmmfft;
+
+namespace {
+
+std::atomic<int> g_last_hip_error{0};
+
+constexpr size_t kAlign = 256;
+inline size_t align_up(size_t v) { return (v + kAlign - 1) / kAlign * kAlign; }
+inline int ilog2(int64_t v) {
+    int r = 0;
+    while (v > 1) { v >>= 1; ++r; }
+    return r;
+}
+inline bool is_pow2(int64_t v) { return v > 0 && (v & (v - 1)) == 0; }
+
+#define HIP_TRY(expr)                                          \
+    do {                                                       \
+        hipError_t e_ = (expr);                                \
+        if (e_ != hipSuccess) {                                \
+            g_last_hip_error.store((int)e_);                   \
+            return e_ == hipErrorOutOfMemory ? FMMFFT_ENOMEM   \
+                                             : FMMFFT_EHIP;    \
+        }                                                      \
+    } while (0)
+
+struct Params {
+    Geom g;
+    int dtype;
+    size_t esize;  // bytes of one real in the plan's precision
+};
+
+int validate(int64_t n, int32_t P, int32_t terms, int32_t leaf, int32_t dtype, Params *out) {
+    if (dtype != FMMFFT_C64 && dtype != FMMFFT_C128) return FMMFFT_EINVAL;
+    if (P < 2 || P > 64 || !is_pow2(P) || n <= 0 || n % P) return FMMFFT_EINVAL;
+    if (terms < 2 || terms > 32 || leaf < 2 || leaf > 128) return FMMFFT_EINVAL;
+    const int64_t M = n / P;
+    if (M % leaf) return FMMFFT_EINVAL;
+    const int64_t s = M / leaf;
+    if (!is_pow2(s) || s < 8) return FMMFFT_EINVAL;
+    if (leaf % 4) return FMMFFT_ENOTSUP;                   // leaf lanes own 4 targets
+    if ((int64_t)leaf * P > (dtype == FMMFFT_C64 ? 8192 : 4096)) return FMMFFT_ENOTSUP;
+    if (!is_pow2(M) || M > (int64_t(1) << 24)) return FMMFFT_ENOTSUP;
+    Geom &g = out->g;
+    g.N = n;
+    g.M = M;
+    g.P = P;
+    g.S = P - 1;
+    g.terms = terms;
+    g.b = leaf;
+    g.s = (int)s;
+    g.n = ilog2(s);
+    out->dtype = dtype;
+    out->esize = dtype == FMMFFT_C64 ? 4 : 8;
+    return FMMFFT_OK;
+}
+
+// Offsets (in doubles) of the FP64 operator tables; the FP32 copies use the
+// same offsets in floats for the first `n_ops` entries.
+struct TableLayout {
+    size_t cheb, initch, ops, topflp, evalm, evalmh, kv, n_ops;  // contiguous block
+    size_t gfac, wP;                                             // complex128 [P]
+    size_t twA, twB, tlo, thi;                                   // complex128
+    size_t total;                                                // doubles
+};
+
+struct FftLayout {
+    bool two_pass;
+    int64_t M1, M2;  // two-pass split (M1 * M2 = M); single pass: M1 = M
+    int logR;
+};
+
+FftLayout fft_layout(const Geom &g) {
+    FftLayout f{};
+    const int m = ilog2(g.M);
+    if (g.M <= 4096) {
+        f.two_pass = false;
+        f.M1 = g.M;
+        f.M2 = 1;
+    } else {
+        f.two_pass = true;
+        f.M1 = int64_t(1) << ((m + 1) / 2);
+        f.M2 = int64_t(1) << (m / 2);
+    }
+    f.logR = (m + 1) / 2;
+    return f;
+}
+
+TableLayout table_layout(const Geom &g, const FftLayout &f) {
+    TableLayout t{};
+    size_t o = 0;
+    const size_t T = g.terms;
+    t.cheb = o; o += T;
+    t.initch = o; o += (size_t)g.b * T;
+    t.ops = o; o += (size_t)(g.n - 2) * NOPS * T * T;
+    t.topflp = o; o += T * T;
+    t.evalm = o; o += (size_t)g.S * g.b * T;
+    t.evalmh = o; o += (size_t)(g.P / 2) * T;
+    t.kv = o; o += (size_t)g.S * 4 * g.b;
+    t.n_ops = o;
+    o = (o + 1) & ~size_t(1);
+    t.gfac = o; o += 2 * (size_t)g.P;
+    t.wP = o; o += 2 * (size_t)g.P;
+    t.twA = o; o += 2 * (size_t)std::max<int64_t>(1, f.M1 / 2);
+    t.twB = o; o += 2 * (size_t)std::max<int64_t>(1, f.M2 / 2);
+    const int64_t R = int64_t(1) << f.logR;
+    t.tlo = o; o += 2 * (size_t)R;
+    t.thi = o; o += 2 * (size_t)std::max<int64_t>(1, g.M / R);
+    t.total = o;
+    return t;
+}
+
+// Launch shapes of the leaf and S2M kernels.
+struct Shapes {
+    int s2m_bpb, s2m_nblk;
+    size_t s2m_lds;
+    int leaf_G, leaf_bb;
+    size_t leaf_lds;
+    int comb_rb;
+    size_t comb_lds;
+    int fftA_C, fftB_C;
+    size_t fftA_lds, fftB_lds;
+};
+
+size_t leaf_lds(const Geom &g, size_t esize, int G, int bb) {
+    return (size_t)(bb + 3) * g.b * G * 2 * esize + (size_t)G * (4 * g.b + 1) * esize +
+           (size_t)(bb + 1) * G * 2 * g.terms * esize;
+}
+
+Shapes shapes(const Geom &g, size_t esize, const FftLayout &f) {
+    Shapes sh{};
+    const size_t lds_cap = 64 * 1024;
+    sh.s2m_bpb = std::max(1, g.s / 2048);
+    sh.s2m_nblk = (g.s + sh.s2m_bpb - 1) / sh.s2m_bpb;
+    sh.s2m_lds = (size_t)g.b * g.P * 2 * esize;
+
+    int G = std::min(g.P, esize == 4 ? 16 : 8);
+    int bb = std::min(g.s, 4);
+    while (leaf_lds(g, esize, G, bb) > lds_cap && bb > 1) bb >>= 1;
+    while (leaf_lds(g, esize, G, bb) > lds_cap && G > 1) G >>= 1;
+    sh.leaf_G = G;
+    sh.leaf_bb = bb;
+    sh.leaf_lds = leaf_lds(g, esize, G, bb);
+
+    sh.comb_rb = (int)std::min<int64_t>(g.M, std::max(1, 2048 / g.P));
+    sh.comb_lds = (size_t)sh.comb_rb * (g.P + 1) * 16;
+
+    const int64_t budget = esize == 4 ? 4096 : 2048;  // complex elements in LDS
+    if (!f.two_pass) {
+        sh.fftA_C = (int)std::min<int64_t>(g.P, std::max<int64_t>(1, budget / f.M1));
+        sh.fftA_lds = (size_t)sh.fftA_C * (f.M1 + 1) * 2 * esize;
+    } else {
+        sh.fftA_C = (int)std::min<int64_t>({16, f.M2, std::max<int64_t>(1, budget / f.M1)});
+        sh.fftA_lds = (size_t)sh.fftA_C * (f.M1 + 1) * 2 * esize;
+        sh.fftB_C = (int)std::min<int64_t>({16, f.M1, std::max<int64_t>(1, budget / f.M2)});
+        sh.fftB_lds = (size_t)sh.fftB_C * (f.M2 + 1) * 2 * esize;
+    }
+    return sh;
+}
+
+struct WsLayout {
+    size_t sums_part, sumsec, phi, psi, psec, stride;
+};
+
+WsLayout ws_layout(const Geom &g, size_t esize, const Shapes &sh) {
+    WsLayout w{};
+    size_t o = 0;
+    w.sums_part = o; o = align_up(o + (size_t)sh.s2m_nblk * g.P * 16);
+    w.sumsec = o; o = align_up(o + (size_t)g.P * 16);
+    const size_t tl = (size_t)tree_len(g) * esize;
+    w.phi = o; o = align_up(o + tl);
+    w.psi = o; o = align_up(o + tl);
+    w.psec = o; o = align_up(o + (size_t)g.N * 2 * esize);
+    w.stride = o;
+    return w;
+}
+
+}  // namespace
+
+struct fmmfft_plan_s {
+    Params prm;
+    FftLayout fft;
+    TableLayout tl;
+    Shapes sh;
+    WsLayout ws;
+    int device;
+    double *tab64 = nullptr;  // FP64 tables (TableLayout offsets)
+    float *tab32 = nullptr;   // FP32 copies of the operator block + twiddles
+    size_t tw32_A = 0, tw32_B = 0;  // float offsets of FP32 stage twiddles
+    void *work = nullptr;     // default workspace
+    int64_t max_batch = 1;
+    // profiling: FMMFFT_NSTAGES + 1 events per recorded execute
+    hipEvent_t *prof_ev = nullptr;
+    int32_t prof_cap = 0;
+    int64_t prof_calls = 0;
+};
+
+namespace {
+
+template <typename T>
+struct TabPtrs {
+    const T *initch, *ops, *topflp, *evalm, *evalmh, *kv, *twA, *twB;
+};
+
+template <typename T>
+TabPtrs<T> tab_ptrs(const fmmfft_plan_s *p);
+
+template <>
+TabPtrs<double> tab_ptrs<double>(const fmmfft_plan_s *p) {
+    const double *t = p->tab64;
+    return {t + p->tl.initch, t + p->tl.ops,   t + p->tl.topflp, t + p->tl.evalm,
+            t + p->tl.evalmh, t + p->tl.kv,    t + p->tl.twA,    t + p->tl.twB};
+}
+template <>
+TabPtrs<float> tab_ptrs<float>(const fmmfft_plan_s *p) {
+    const float *t = p->tab32;
+    return {t + p->tl.initch, t + p->tl.ops,   t + p->tl.topflp, t + p->tl.evalm,
+            t + p->tl.evalmh, t + p->tl.kv,    t + p->tw32_A,    t + p->tw32_B};
+}
+
+inline unsigned blocks_for(int64_t n, int per = 256) { return (unsigned)((n + per - 1) / per); }
+
+template <typename T>
+int run(const fmmfft_plan_s *p, int64_t count, const void *d_in, void *d_out, int dir, char *ws,
+        hipStream_t st) {
+    const Geom &g = p->prm.g;
+    const Shapes &sh = p->sh;
+    const WsLayout &wl = p->ws;
+    const TabPtrs<T> tb = tab_ptrs<T>(p);
+    const int conj = dir < 0 ? 1 : 0;
+    const unsigned nz = (unsigned)count;
+    const int64_t wstride_T = (int64_t)(wl.stride / sizeof(T));
+    const int64_t wstride_c64 = (int64_t)(wl.stride / sizeof(c64));
+    const int64_t wstride_cx = (int64_t)(wl.stride / sizeof(cx<T>));
+
+    const cx<T> *x = static_cast<const cx<T> *>(d_in);
+    cx<T> *out = static_cast<cx<T> *>(d_out);
+    T *phi = reinterpret_cast<T *>(ws + wl.phi);
+    T *psi = reinterpret_cast<T *>(ws + wl.psi);
+    c64 *sums_part = reinterpret_cast<c64 *>(ws + wl.sums_part);
+    c64 *sumsec = reinterpret_cast<c64 *>(ws + wl.sumsec);
+    cx<T> *psec = reinterpret_cast<cx<T> *>(ws + wl.psec);
+    const c64 *gfac = reinterpret_cast<const c64 *>(p->tab64 + p->tl.gfac);
+    const c64 *wP = reinterpret_cast<const c64 *>(p->tab64 + p->tl.wP);
+    const c64 *tlo = reinterpret_cast<const c64 *>(p->tab64 + p->tl.tlo);
+    const c64 *thi = reinterpret_cast<const c64 *>(p->tab64 + p->tl.thi);
+
+    hipEvent_t *ev = nullptr;
+    if (p->prof_ev && p->prof_calls < p->prof_cap) {
+        ev = p->prof_ev + (size_t)p->prof_calls * (FMMFFT_NSTAGES + 1);
+        const_cast<fmmfft_plan_s *>(p)->prof_calls++;
+    }
+    auto mark = [&](int i) {
+        if (ev) (void)hipEventRecord(ev[i], st);
+    };
+    mark(0);
+    hipLaunchKernelGGL(k_s2m<T>, dim3(sh.s2m_nblk, 1, nz), dim3(256), sh.s2m_lds, st, g, x, g.N, phi,
+                       wstride_T, sums_part, wstride_c64, tb.initch, sh.s2m_bpb, conj);
+    hipLaunchKernelGGL(k_sums_reduce, dim3(g.P, 1, nz), dim3(256), 0, st, g, sums_part, sh.s2m_nblk,
+                       wstride_c64, sumsec, wstride_c64);
+    mark(1);
+    const int64_t node_len = (int64_t)g.S * 2 * g.terms;
+    for (int l = g.n - 1; l >= 2; --l)
+        hipLaunchKernelGGL(k_m2m<T>, dim3(blocks_for((int64_t(1) << l) * node_len), 1, nz), dim3(256), 0, st,
+                           g, l, phi, wstride_T, tb.ops);
+    mark(2);
+    for (int l = 2; l <= g.n; ++l)
+        hipLaunchKernelGGL(k_down<T>, dim3(blocks_for((int64_t(1) << l) * node_len), 1, nz), dim3(256), 0, st,
+                           g, l, phi, psi, wstride_T, tb.ops, tb.topflp);
+    mark(3);
+    hipLaunchKernelGGL(k_leaf<T>, dim3(g.s / sh.leaf_bb, g.P / sh.leaf_G, nz), dim3(256), sh.leaf_lds, st, g,
+                       x, g.N, psi, wstride_T, psec, wstride_cx, tb.evalm, tb.evalmh, tb.kv, sh.leaf_G,
+                       sh.leaf_bb, conj);
+    mark(4);
+    hipLaunchKernelGGL(k_combine<T>, dim3((unsigned)(g.M / sh.comb_rb), 1, nz), dim3(256), sh.comb_lds, st, g,
+                       psec, wstride_cx, sumsec, wstride_c64, gfac, wP, out, g.N, sh.comb_rb);
+    mark(5);
+
+    const FftLayout &f = p->fft;
+    if (!f.two_pass) {
+        FftPass fp{};
+        fp.L = (int)f.M1;
+        fp.logL = ilog2(f.M1);
+        fp.C = sh.fftA_C;
+        fp.F = 1;
+        fp.in_fs = g.M; fp.in_es = 1;
+        fp.out_fs = g.M; fp.out_ks = 1;
+        fp.twiddle = 0;
+        fp.conj_out = conj;
+        fp.logR = f.logR;
+        hipLaunchKernelGGL(k_fft<T>, dim3((unsigned)(g.P / fp.C), 1, nz), dim3(256), sh.fftA_lds, st, g, fp,
+                           out, out, g.N, g.N, tb.twA, tlo, thi);
+        mark(6);
+    } else {
+        FftPass a{};
+        a.L = (int)f.M1;
+        a.logL = ilog2(f.M1);
+        a.C = sh.fftA_C;
+        a.F = f.M2;
+        a.in_fs = 1; a.in_es = f.M2;
+        a.out_fs = 1; a.out_ks = f.M2;
+        a.twiddle = 1;
+        a.conj_out = 0;
+        a.logR = f.logR;
+        hipLaunchKernelGGL(k_fft<T>, dim3((unsigned)(g.P * f.M2 / a.C), 1, nz), dim3(256), sh.fftA_lds, st, g, a,
+                           out, psec, g.N, wstride_cx, tb.twA, tlo, thi);
+        mark(6);
+        FftPass bq{};
+        bq.L = (int)f.M2;
+        bq.logL = ilog2(f.M2);
+        bq.C = sh.fftB_C;
+        bq.F = f.M1;
+        bq.in_fs = f.M2; bq.in_es = 1;
+        bq.out_fs = 1; bq.out_ks = f.M1;
+        bq.twiddle = 0;
+        bq.conj_out = conj;
+        bq.logR = f.logR;
+        hipLaunchKernelGGL(k_fft<T>, dim3((unsigned)(g.P * f.M1 / bq.C), 1, nz), dim3(256), sh.fftB_lds, st, g,
+                           bq, psec, out, wstride_cx, g.N, tb.twB, tlo, thi);
+    }
+    mark(7);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        g_last_hip_error.store((int)e);
+        return FMMFFT_EHIP;
+    }
+    return FMMFFT_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+
+extern "C" {
+
+int fmmfft_plan_sizes(int64_t n, int32_t p_blocks, int32_t terms, int32_t leaf, int32_t dtype,
+                      size_t *table_bytes, size_t *work_bytes) {
+    Params prm{};
+    int rc = validate(n, p_blocks, terms, leaf, dtype, &prm);
+    if (rc != FMMFFT_OK) return rc;
+    const FftLayout f = fft_layout(prm.g);
+    const TableLayout tl = table_layout(prm.g, f);
+    const Shapes sh = shapes(prm.g, prm.esize, f);
+    const WsLayout wl = ws_layout(prm.g, prm.esize, sh);
+    if (table_bytes) {
+        size_t t = tl.total * sizeof(double);
+        if (dtype == FMMFFT_C64) t += (tl.n_ops + f.M1 + f.M2 + 4) * sizeof(float);
+        *table_bytes = t;
+    }
+    if (work_bytes) *work_bytes = wl.stride;
+    return FMMFFT_OK;
+}
+
+int fmmfft_plan_create(fmmfft_plan *out, int64_t n, int32_t p_blocks, int32_t terms, int32_t leaf,
+                       int32_t dtype, int32_t max_batch) {
+    if (!out) return FMMFFT_EINVAL;
+    *out = nullptr;
+    if (max_batch < 0) return FMMFFT_EINVAL;
+    Params prm{};
+    int rc = validate(n, p_blocks, terms, leaf, dtype, &prm);
+    if (rc != FMMFFT_OK) return rc;
+    fmmfft_plan_s *p = new (std::nothrow) fmmfft_plan_s();
+    if (!p) return FMMFFT_ENOMEM;
+    p->prm = prm;
+    p->fft = fft_layout(prm.g);
+    p->tl = table_layout(prm.g, p->fft);
+    p->sh = shapes(prm.g, prm.esize, p->fft);
+    p->ws = ws_layout(prm.g, prm.esize, p->sh);
+    p->max_batch = max_batch == 0 ? 1 : max_batch;
+    const Geom &g = prm.g;
+
+    auto fail = [&](int code) {
+        fmmfft_plan_destroy(p);
+        return code;
+    };
+#define PLAN_TRY(expr)                                                                 \
+    do {                                                                               \
+        hipError_t e_ = (expr);                                                        \
+        if (e_ != hipSuccess) {                                                        \
+            g_last_hip_error.store((int)e_);                                           \
+            return fail(e_ == hipErrorOutOfMemory ? FMMFFT_ENOMEM : FMMFFT_EHIP);      \
+        }                                                                              \
+    } while (0)
+
+    PLAN_TRY(hipGetDevice(&p->device));
+    PLAN_TRY(hipMalloc(&p->tab64, p->tl.total * sizeof(double)));
+    hipStream_t st;
+    PLAN_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    double *t = p->tab64;
+    const TableLayout &tl = p->tl;
+    const int64_t n_closed = g.terms + (int64_t)g.b * g.terms + (int64_t)g.S * 4 * g.b + g.P;
+    hipLaunchKernelGGL(k_build_closed, dim3(blocks_for(n_closed)), dim3(256), 0, st, g, t + tl.cheb,
+                       t + tl.initch, t + tl.kv, reinterpret_cast<double2 *>(t + tl.gfac),
+                       reinterpret_cast<double2 *>(t + tl.wP));
+    const int jobs = 8 * (g.n - 2) + 1 + g.S * g.b + g.P / 2;
+    hipLaunchKernelGGL(k_build_interp, dim3(jobs), dim3(64), 0, st, g, t + tl.ops, t + tl.topflp,
+                       t + tl.evalm, t + tl.evalmh);
+    const int64_t R = int64_t(1) << p->fft.logR;
+    const int64_t MA = p->fft.M1, MB = std::max<int64_t>(2, p->fft.M2);
+    hipLaunchKernelGGL(k_build_twiddles, dim3(blocks_for(MA / 2 + R + g.M / R)), dim3(256), 0, st, MA, g.M, R,
+                       reinterpret_cast<double2 *>(t + tl.twA), reinterpret_cast<double2 *>(t + tl.tlo),
+                       reinterpret_cast<double2 *>(t + tl.thi));
+    if (p->fft.two_pass)
+        hipLaunchKernelGGL(k_build_twiddles, dim3(blocks_for(MB / 2)), dim3(256), 0, st, MB, g.M, int64_t(0),
+                           reinterpret_cast<double2 *>(t + tl.twB), reinterpret_cast<double2 *>(t + tl.tlo),
+                           reinterpret_cast<double2 *>(t + tl.thi));
+    if (prm.dtype == FMMFFT_C64) {
+        const size_t nA = 2 * (size_t)std::max<int64_t>(1, MA / 2), nB = 2 * (size_t)std::max<int64_t>(1, MB / 2);
+        p->tw32_A = tl.n_ops;
+        p->tw32_B = tl.n_ops + nA;
+        const size_t n32 = tl.n_ops + nA + nB;
+        hipError_t e = hipMalloc(&p->tab32, n32 * sizeof(float));
+        if (e != hipSuccess) {
+            (void)hipStreamDestroy(st);
+            g_last_hip_error.store((int)e);
+            return fail(e == hipErrorOutOfMemory ? FMMFFT_ENOMEM : FMMFFT_EHIP);
+        }
+        hipLaunchKernelGGL(k_to_f32, dim3(blocks_for(tl.n_ops)), dim3(256), 0, st, t, p->tab32, (int64_t)tl.n_ops);
+        hipLaunchKernelGGL(k_to_f32, dim3(blocks_for(nA)), dim3(256), 0, st, t + tl.twA, p->tab32 + p->tw32_A,
+                           (int64_t)nA);
+        if (p->fft.two_pass)
+            hipLaunchKernelGGL(k_to_f32, dim3(blocks_for(nB)), dim3(256), 0, st, t + tl.twB,
+                               p->tab32 + p->tw32_B, (int64_t)nB);
+    }
+    hipError_t e1 = hipGetLastError();
+    hipError_t e2 = hipStreamSynchronize(st);
+    (void)hipStreamDestroy(st);
+    PLAN_TRY(e1);
+    PLAN_TRY(e2);
+    PLAN_TRY(hipMalloc(&p->work, p->ws.stride * (size_t)p->max_batch));
+#undef PLAN_TRY
+    *out = p;
+    return FMMFFT_OK;
+}
+
+static void prof_free(fmmfft_plan plan) {
+    if (plan->prof_ev) {
+        for (size_t i = 0; i < (size_t)plan->prof_cap * (FMMFFT_NSTAGES + 1); ++i)
+            (void)hipEventDestroy(plan->prof_ev[i]);
+        delete[] plan->prof_ev;
+    }
+    plan->prof_ev = nullptr;
+    plan->prof_cap = 0;
+    plan->prof_calls = 0;
+}
+
+int fmmfft_profile_begin(fmmfft_plan plan, int32_t max_calls) {
+    if (!plan || max_calls <= 0 || max_calls > 100000) return FMMFFT_EINVAL;
+    prof_free(plan);
+    const size_t n = (size_t)max_calls * (FMMFFT_NSTAGES + 1);
+    plan->prof_ev = new (std::nothrow) hipEvent_t[n];
+    if (!plan->prof_ev) return FMMFFT_ENOMEM;
+    for (size_t i = 0; i < n; ++i) {
+        hipError_t e = hipEventCreate(&plan->prof_ev[i]);
+        if (e != hipSuccess) {
+            for (size_t j = 0; j < i; ++j) (void)hipEventDestroy(plan->prof_ev[j]);
+            delete[] plan->prof_ev;
+            plan->prof_ev = nullptr;
+            g_last_hip_error.store((int)e);
+            return FMMFFT_EHIP;
+        }
+    }
+    plan->prof_cap = max_calls;
+    plan->prof_calls = 0;
+    return FMMFFT_OK;
+}
+
+int fmmfft_profile_end(fmmfft_plan plan, double *stage_ms, int32_t nstages, int64_t *calls) {
+    if (!plan || !plan->prof_ev) return FMMFFT_EINVAL;
+    const int ns = std::min<int32_t>(nstages, FMMFFT_NSTAGES);
+    for (int i = 0; stage_ms && i < ns; ++i) stage_ms[i] = 0.0;
+    int rc = FMMFFT_OK;
+    for (int64_t c = 0; c < plan->prof_calls; ++c) {
+        hipEvent_t *ev = plan->prof_ev + (size_t)c * (FMMFFT_NSTAGES + 1);
+        hipError_t e = hipEventSynchronize(ev[FMMFFT_NSTAGES]);
+        if (e != hipSuccess) {
+            g_last_hip_error.store((int)e);
+            rc = FMMFFT_EHIP;
+            break;
+        }
+        for (int i = 0; stage_ms && i < ns; ++i) {
+            float ms = 0.f;
+            if (i == 6 && !plan->fft.two_pass) continue;  // single-pass FFT: stage 6 is empty
+            if (hipEventElapsedTime(&ms, ev[i], ev[i + 1]) == hipSuccess) stage_ms[i] += ms;
+        }
+    }
+    if (calls) *calls = plan->prof_calls;
+    prof_free(plan);
+    return rc;
+}
+
+int fmmfft_plan_destroy(fmmfft_plan plan) {
+    if (!plan) return FMMFFT_EINVAL;
+    prof_free(plan);
+    if (plan->tab64) (void)hipFree(plan->tab64);
+    if (plan->tab32) (void)hipFree(plan->tab32);
+    if (plan->work) (void)hipFree(plan->work);
+    delete plan;
+    return FMMFFT_OK;
+}
+
+size_t fmmfft_workspace_bytes(fmmfft_plan plan, int64_t count) {
+    if (!plan || count <= 0) return 0;
+    return plan->ws.stride * (size_t)count;
+}
+
+int fmmfft_execute_ws(fmmfft_plan plan, int64_t count, const void *d_in, void *d_out, int32_t dir,
+                      void *d_work, size_t work_bytes, void *stream) {
+    if (!plan || !d_in || !d_out || !d_work || count <= 0 || count > 65535) return FMMFFT_EINVAL;
+    if (dir != FMMFFT_FORWARD && dir != FMMFFT_INVERSE) return FMMFFT_EINVAL;
+    if (work_bytes < plan->ws.stride * (size_t)count || ((uintptr_t)d_work % kAlign) != 0) return FMMFFT_EINVAL;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    if (plan->prm.dtype == FMMFFT_C64) return run<float>(plan, count, d_in, d_out, dir, (char *)d_work, st);
+    return run<double>(plan, count, d_in, d_out, dir, (char *)d_work, st);
+}
+
+int fmmfft_execute_batched(fmmfft_plan plan, int64_t count, const void *d_in, void *d_out, int32_t dir,
+                           void *stream) {
+    if (!plan || count <= 0 || count > plan->max_batch) return FMMFFT_EINVAL;
+    return fmmfft_execute_ws(plan, count, d_in, d_out, dir, plan->work, plan->ws.stride * (size_t)count,
+                             stream);
+}
+
+int fmmfft_execute(fmmfft_plan plan, const void *d_in, void *d_out, int32_t dir, void *stream) {
+    return fmmfft_execute_batched(plan, 1, d_in, d_out, dir, stream);
+}
+
+int fmmfft_plan_get(fmmfft_plan plan, int64_t *n, int32_t *p_blocks, int32_t *terms, int32_t *leaf,
+                    int32_t *dtype, int32_t *depth) {
+    if (!plan) return FMMFFT_EINVAL;
+    const Geom &g = plan->prm.g;
+    if (n) *n = g.N;
+    if (p_blocks) *p_blocks = g.P;
+    if (terms) *terms = g.terms;
+    if (leaf) *leaf = g.b;
+    if (dtype) *dtype = plan->prm.dtype;
+    if (depth) *depth = g.n;
+    return FMMFFT_OK;
+}
+
+int fmmfft_read_table(fmmfft_plan plan, int32_t which, void *host_dst, size_t bytes) {
+    if (!plan || !host_dst) return FMMFFT_EINVAL;
+    const Geom &g = plan->prm.g;
+    const TableLayout &tl = plan->tl;
+    size_t off, cnt;
+    switch (which) {
+        case 0: off = tl.cheb; cnt = g.terms; break;
+        case 1: off = tl.initch; cnt = (size_t)g.b * g.terms; break;
+        case 2: off = tl.ops; cnt = (size_t)(g.n - 2) * NOPS * g.terms * g.terms; break;
+        case 3: off = tl.topflp; cnt = (size_t)g.terms * g.terms; break;
+        case 4: off = tl.evalm; cnt = (size_t)g.S * g.b * g.terms; break;
+        case 5: off = tl.evalmh; cnt = (size_t)(g.P / 2) * g.terms; break;
+        case 6: off = tl.kv; cnt = (size_t)g.S * 4 * g.b; break;
+        case 7: off = tl.gfac; cnt = 2 * (size_t)g.P; break;
+        default: return FMMFFT_EINVAL;
+    }
+    if (bytes != cnt * sizeof(double)) return FMMFFT_EINVAL;
+    HIP_TRY(hipMemcpy(host_dst, plan->tab64 + off, bytes, hipMemcpyDeviceToHost));
+    return FMMFFT_OK;
+}
+
+int fmmfft_read_stage(fmmfft_plan plan, int32_t which, void *host_dst, size_t bytes) {
+    if (!plan || !host_dst) return FMMFFT_EINVAL;
+    const Geom &g = plan->prm.g;
+    const WsLayout &wl = plan->ws;
+    size_t off, cnt;
+    switch (which) {
+        case 0: off = wl.sumsec; cnt = (size_t)g.P * 16; break;
+        case 1: off = wl.phi; cnt = (size_t)tree_len(g) * plan->prm.esize; break;
+        case 2: off = wl.psi; cnt = (size_t)tree_len(g) * plan->prm.esize; break;
+        case 3: off = wl.psec; cnt = (size_t)g.N * 2 * plan->prm.esize; break;
+        default: return FMMFFT_EINVAL;
+    }
+    if (bytes != cnt) return FMMFFT_EINVAL;
+    HIP_TRY(hipDeviceSynchronize());
+    HIP_TRY(hipMemcpy(host_dst, (char *)plan->work + off, bytes, hipMemcpyDeviceToHost));
+    return FMMFFT_OK;
+}
+
+const char *fmmfft_strerror(int status) {
+    switch (status) {
+        case FMMFFT_OK: return "ok";
+        case FMMFFT_EINVAL: return "invalid argument";
+        case FMMFFT_ENOMEM: return "out of device memory";
+        case FMMFFT_EHIP: return "HIP runtime error";
+        case FMMFFT_ENOTSUP: return "parameters not supported by the kernels";
+        default: return "unknown status";
+    }
+}
+
+int fmmfft_last_hip_error(void) { return g_last_hip_error.load(); }
+
+}  // extern "C"

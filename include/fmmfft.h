@@ -1,0 +1,141 @@
+/*
+ * fmmfft.h -- C ABI of the MI355X-native FMM-FFT (libfmmfft_amd.so).
+ *
+ * Forward (or unnormalised inverse) 1-D complex DFT of length N = P*M by the
+ * Edelman-McCorquodale-Toledo FMM-FFT: the signal is split into P polyphase
+ * sections, the P-1 fractional-shift (cotangent-kernel) matvecs are applied
+ * with a 1-D Chebyshev fast multipole method, the sections are combined by a
+ * P-point DFT and P length-M FFTs finish the transform.  All work runs as
+ * hand-written gfx950 HIP kernels on device-resident buffers.
+ *
+ * The entry points replace the reference's plan/execute pair:
+ *   fmmfft_plan_create  <- mfti()      src/fmmfft.c:337-382 (+ mftii :261-332)
+ *   fmmfft_execute      <- mft()       src/fmmfft.c:980-1081 (mftint :384-978
+ *                                      and fftw_execute :1080)
+ *   fmmfft_plan_sizes   <- getwlen2() / getvlen()  src/fmmfft.c:1110-1183
+ * The reference's P MPI ranks are the P blocks of one device buffer: the
+ * caller passes all N points at once, in natural order, and receives all N
+ * outputs in natural order (block r of the output is what rank r held after
+ * mft()).  There is no myid, no MPI communicator and no caller-carved
+ * workspace: the plan owns its tables and its default workspace.
+ *
+ * Every function returns an fmmfft_status; none aborts.  Executes are
+ * stream-ordered and never synchronise the host.  One plan may be used by one
+ * stream at a time with its own workspace; concurrent executes of the same
+ * plan must pass distinct workspaces to fmmfft_execute_ws().
+ */
+#ifndef FMMFFT_H
+#define FMMFFT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct fmmfft_plan_s *fmmfft_plan;
+
+/* Storage / arithmetic type of a plan. */
+enum fmmfft_dtype {
+    FMMFFT_C64 = 0,  /* complex64 storage, FP32 contractions, FP64 setup,
+                        section sums, combine and inter-pass twiddles    */
+    FMMFFT_C128 = 1  /* complex128 end to end                           */
+};
+
+enum fmmfft_status {
+    FMMFFT_OK = 0,
+    FMMFFT_EINVAL = 1,  /* invalid (N, P, terms, leaf), dtype, pointer or count */
+    FMMFFT_ENOMEM = 2,  /* a device allocation failed                          */
+    FMMFFT_EHIP = 3,    /* a HIP runtime call failed (see fmmfft_last_hip_error) */
+    FMMFFT_ENOTSUP = 4  /* valid parameters outside what the kernels support    */
+};
+
+/* Direction: +1 forward X[k] = sum x[n] e^{-2 pi i nk/N}; -1 unnormalised
+ * inverse (conj(F(conj x))).  The reference only runs dir = +1
+ * (src/fmmfft.c:1260); its dir < 0 branch conjugates before its local FFT
+ * (src/fmmfft.c:976-977 vs :1080) and is not a correct inverse. */
+#define FMMFFT_FORWARD 1
+#define FMMFFT_INVERSE (-1)
+
+/* Sizes of a would-be plan without creating it (replaces getwlen2 /
+ * getvlen, src/fmmfft.c:1110-1183).  table_bytes: device operator tables;
+ * work_bytes: device workspace for ONE signal.  Either pointer may be NULL.
+ * Validity rules (the reference checks none of them, src/fmmfft.c:1200-1212):
+ * N % P == 0, P a power of two with 2 <= P <= 64, M = N/P divisible by leaf,
+ * s = M/leaf a power of two >= 8, 2 <= terms <= 32, 2 <= leaf <= 128,
+ * leaf * P <= 8192 (complex64) / 4096 (complex128). */
+int fmmfft_plan_sizes(int64_t n, int32_t p_blocks, int32_t terms, int32_t leaf,
+                      int32_t dtype, size_t *table_bytes, size_t *work_bytes);
+
+/* Build a plan on the current HIP device (replaces mfti, src/fmmfft.c:337):
+ * n = N, p_blocks = P (the reference's MPI world size), terms = Chebyshev
+ * order (the reference's T), leaf = points per leaf box (the reference's B;
+ * tree depth = log2(N / (P * leaf))).  All operator tables are built on the
+ * device in FP64 (plus FP32 copies for FMMFFT_C64).  max_batch sizes the
+ * plan's default workspace (signals per fmmfft_execute_batched call);
+ * 0 means 1. */
+int fmmfft_plan_create(fmmfft_plan *out, int64_t n, int32_t p_blocks, int32_t terms,
+                       int32_t leaf, int32_t dtype, int32_t max_batch);
+
+int fmmfft_plan_destroy(fmmfft_plan plan);
+
+/* One transform (replaces mft, src/fmmfft.c:980).  d_in, d_out: device
+ * pointers to N complex values (complex64 or complex128 per the plan),
+ * natural order; d_in == d_out is allowed.  stream: a hipStream_t (NULL =
+ * default stream). */
+int fmmfft_execute(fmmfft_plan plan, const void *d_in, void *d_out, int32_t dir,
+                   void *stream);
+
+/* count independent signals, signal i at d_in + i*N (and d_out + i*N), one
+ * launch sequence; count <= the plan's max_batch. */
+int fmmfft_execute_batched(fmmfft_plan plan, int64_t count, const void *d_in,
+                           void *d_out, int32_t dir, void *stream);
+
+/* Bytes of caller workspace for count signals, and the execute that uses
+ * it (d_work must be 256-byte aligned device memory of that size). */
+size_t fmmfft_workspace_bytes(fmmfft_plan plan, int64_t count);
+int fmmfft_execute_ws(fmmfft_plan plan, int64_t count, const void *d_in, void *d_out,
+                      int32_t dir, void *d_work, size_t work_bytes, void *stream);
+
+/* Plan parameters (any pointer may be NULL). */
+int fmmfft_plan_get(fmmfft_plan plan, int64_t *n, int32_t *p_blocks, int32_t *terms,
+                    int32_t *leaf, int32_t *dtype, int32_t *depth);
+
+/* Copy a device operator table (always FP64) to host memory, for parity
+ * checks.  which: 0 cheb [terms], 1 initch [leaf][terms], 2 level operators
+ * [depth-2][8][terms][terms] (shftfn, shftfp, flip2n, flip2p, flip3n, flip3p,
+ * shftln, shftlp), 3 topflp [terms][terms], 4 evalm [P-1][leaf][terms],
+ * 5 evalmh [P/2][terms], 6 near kernel [P-1][4*leaf], 7 combine factors
+ * [P] complex (re, im).  bytes must equal the table's size. */
+int fmmfft_read_table(fmmfft_plan plan, int32_t which, void *host_dst, size_t bytes);
+
+/* Copy an intermediate of the most recent execute that used the plan's
+ * default workspace (signal 0) to host memory; synchronises the device.
+ * which: 0 section sums [P] complex128 ((i/M) * sum, entry 0 = 0),
+ * 1 multipole tree, 2 local tree (both levels 2..depth, node-major,
+ * [node][P-1][2][terms] real in the plan's precision),
+ * 3 section cot sums psi [M][P] (column 0 holds x_0) in the plan's complex
+ * type.  bytes must equal the buffer's size. */
+int fmmfft_read_stage(fmmfft_plan plan, int32_t which, void *host_dst, size_t bytes);
+
+/* Per-stage timing with HIP events recorded on each execute's own stream,
+ * for benchmarks (no reference counterpart; the reference times whole mft()
+ * calls with omp_get_wtime, src/fmmfft.c:1259-1263).  Stages:
+ * 0 S2M + section sums, 1 upward pass, 2 downward pass, 3 leaf evaluation
+ * (L2T + near field), 4 combine, 5 local FFT pass 1, 6 local FFT pass 2.
+ * profile_begin creates events for up to max_calls executes; profile_end
+ * synchronises, writes the summed milliseconds of each stage over the
+ * recorded executes and their count, and stops recording. */
+#define FMMFFT_NSTAGES 7
+int fmmfft_profile_begin(fmmfft_plan plan, int32_t max_calls);
+int fmmfft_profile_end(fmmfft_plan plan, double *stage_ms, int32_t nstages, int64_t *calls);
+
+const char *fmmfft_strerror(int status);
+/* The last failing HIP error code seen by the library (0 if none). */
+int fmmfft_last_hip_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,265 @@
+"""GPU parity of the HIP FMM-FFT (called through the C ABI) against the CPU
+oracle: operator tables, every FMM stage, the whole transform at small sizes
+and at the full benchmark sizes, and the transform identities.
+
+Tolerances: complex128 plans must match the oracle's FMM stages to ~1e-12
+and an FP64 FFT to rel-L2 <= 1e-11; complex64 plans must reach rel-L2 <= 1e-6
+against the FP64 FFT of the same (complex64-rounded) input.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+
+
+@pytest.fixture(scope="module")
+def torch():
+    import torch
+
+    assert torch.cuda.is_available(), "GPU tests need a GPU"
+    return torch
+
+
+def to_dev(torch, x, dtype):
+    return torch.from_numpy(np.ascontiguousarray(x)).to(dtype).cuda()
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def run_plan(fmm, torch, x, n, p, terms, leaf, dtype, direction=1):
+    plan = fmm.Plan(n, p, terms, leaf, dtype)
+    xd = to_dev(torch, x, plan.torch_dtype)
+    out = plan.execute(xd, direction=direction)
+    torch.cuda.synchronize()
+    return plan, out.cpu().numpy().astype(np.complex128)
+
+
+# ---------------------------------------------------------------------------
+# tables and stages (complex128)
+# ---------------------------------------------------------------------------
+
+@pytest.mark.parametrize("n,p,terms,leaf", [(1 << 14, 4, 16, 32), (1 << 15, 8, 20, 64), (1 << 13, 2, 12, 16)])
+def test_tables_match_oracle(fmm, orc, torch, n, p, terms, leaf):
+    say(f"tables n={n} p={p} terms={terms} leaf={leaf}")
+    plan = fmm.Plan(n, p, terms, leaf, fmm.C128)
+    ref = orc.OraclePlan(n, p, terms, leaf)
+    S, b = p - 1, leaf
+    tabs = [
+        (0, ref.cheb),
+        (1, ref.initch),
+        (2, ref.ops),
+        (3, ref.topflp),
+        (4, ref.evalm),
+        (5, ref.evalmh),
+        (6, ref.kv),
+    ]
+    for which, expect in tabs:
+        got = plan.read_table(which, expect.size).reshape(expect.shape)
+        err = np.max(np.abs(got - expect)) / np.max(np.abs(expect))
+        assert err < 1e-12, (which, err)
+    g = plan.read_table(7, 2 * p)
+    gfac = g[0::2] + 1j * g[1::2]
+    assert np.max(np.abs(gfac[1:] - ref.fo[0])) < 1e-14
+    assert S == ref.kv.shape[0] and ref.kv.shape[1] == 4 * b
+
+
+@pytest.mark.parametrize("n,p,terms,leaf", [(1 << 14, 4, 16, 32), (1 << 15, 8, 20, 64), (1 << 13, 16, 12, 16)])
+def test_stages_match_oracle_fp64(fmm, orc, torch, n, p, terms, leaf):
+    say(f"stages n={n} p={p} terms={terms} leaf={leaf}")
+    x = orc.normal(n, 7)
+    plan, _ = run_plan(fmm, torch, x, n, p, terms, leaf, fmm.C128)
+    ref = orc.OraclePlan(n, p, terms, leaf)
+    S, M = p - 1, n // p
+    ss = ref.section_sums(x)
+    g_ss = plan.read_stage(0, p, np.complex128)
+    assert rel(g_ss[1:], ss[1:]) < 1e-13 and g_ss[0] == 0
+    phi = ref.upward(x)
+    psi = ref.downward(phi)
+    tl = ref.tree_len()
+    for which, expect in ((1, phi), (2, psi)):
+        raw = plan.read_stage(which, 2 * tl, np.float64).reshape(-1, S, 2, terms)
+        got = (raw[:, :, 0, :] + 1j * raw[:, :, 1, :]).reshape(-1)
+        assert rel(got, expect) < 1e-13, which
+    ps = ref.evaluate(x, psi).reshape(M, p)
+    g_ps = plan.read_stage(3, n, np.complex128).reshape(M, p)
+    assert rel(g_ps[:, 1:], ps[:, 1:]) < 1e-13
+    assert np.array_equal(g_ps[:, 0], x.reshape(M, p)[:, 0])
+
+
+# ---------------------------------------------------------------------------
+# whole transform, small sizes
+# ---------------------------------------------------------------------------
+
+SMALL = [
+    (1 << 12, 2, 12, 32),   # P = 2, single-pass local FFT
+    (1 << 14, 4, 16, 32),
+    (1 << 15, 8, 20, 64),
+    (1 << 16, 16, 12, 32),  # M = 4096
+    (1 << 17, 8, 16, 64),   # two-pass local FFT
+    (1 << 18, 64, 16, 32),
+]
+
+
+@pytest.mark.parametrize("n,p,terms,leaf", SMALL)
+def test_fp64_transform(fmm, orc, torch, n, p, terms, leaf):
+    say(f"fp64 n={n} p={p} terms={terms} leaf={leaf}")
+    x = orc.normal(n, 11)
+    _, X = run_plan(fmm, torch, x, n, p, terms, leaf, fmm.C128)
+    ref = orc.OraclePlan(n, p, terms, leaf).fmmfft(x)
+    Y = orc.fft64(x)
+    assert rel(X, ref) < 1e-12
+    assert rel(X, Y) < 1e-11
+
+
+@pytest.mark.parametrize("n,p,terms,leaf", SMALL)
+def test_fp32_transform(fmm, orc, torch, n, p, terms, leaf):
+    say(f"fp32 n={n} p={p} terms={terms} leaf={leaf}")
+    x = orc.normal(n, 12).astype(np.complex64).astype(np.complex128)
+    _, X = run_plan(fmm, torch, x, n, p, terms, leaf, fmm.C64)
+    Y = orc.fft64(x)
+    err = rel(X, Y)
+    say(f"  rel-L2 {err:.3e}")
+    assert err < 1e-6
+
+
+def test_cpu_config_on_gpu(fmm, orc, torch):
+    """N=2^16 complex128, P=4, p=16, leaf 64, uniform[-1,1] seed 1."""
+    say("cpu config on gpu")
+    n = 1 << 16
+    x = orc.uniform(n, 1)
+    _, X = run_plan(fmm, torch, x, n, 4, 16, 64, fmm.C128)
+    assert rel(X, orc.fft64(x)) < 1e-11
+    assert orc.max_abs(X, orc.fft64(x)) / np.abs(orc.fft64(x)).max() < 1e-11
+
+
+@pytest.mark.parametrize("dtype,tol", [(1, 1e-11), (0, 1e-6)])
+def test_inverse_round_trip(fmm, orc, torch, dtype, tol):
+    say(f"inverse dtype={dtype}")
+    n, p, terms, leaf = 1 << 15, 8, 16, 32
+    x = orc.normal(n, 21)
+    if dtype == 0:
+        x = x.astype(np.complex64).astype(np.complex128)
+    plan = fmm.Plan(n, p, terms, leaf, dtype)
+    xd = to_dev(torch, x, plan.torch_dtype)
+    X = plan.execute(xd)
+    xr = plan.execute(X, direction=fmm.INVERSE)
+    Xi = plan.execute(xd, direction=fmm.INVERSE)
+    torch.cuda.synchronize()
+    assert rel(xr.cpu().numpy() / n, x) < tol
+    assert rel(Xi.cpu().numpy(), orc.fft64(x, +1)) < tol
+
+
+def test_in_place_equals_out_of_place(fmm, orc, torch):
+    say("in place")
+    n, p = 1 << 16, 8
+    x = orc.normal(n, 5)
+    plan = fmm.Plan(n, p, 16, 32, fmm.C128)
+    xd = to_dev(torch, x, plan.torch_dtype)
+    out = plan.execute(xd.clone())
+    fmm.mft(plan, xd)
+    torch.cuda.synchronize()
+    assert torch.equal(out, xd)
+
+
+@pytest.mark.parametrize("dtype", [0, 1])
+def test_batched_equals_single(fmm, orc, torch, dtype):
+    say(f"batched dtype={dtype}")
+    n, p, count = 1 << 15, 8, 5
+    xs = np.stack([orc.normal(n, 100 + i) for i in range(count)])
+    plan = fmm.Plan(n, p, 16, 32, dtype, max_batch=count)
+    xd = to_dev(torch, xs, plan.torch_dtype)
+    batched = plan.execute(xd)
+    singles = [plan.execute(xd[i].contiguous()) for i in range(count)]
+    torch.cuda.synchronize()
+    for i in range(count):
+        assert torch.equal(batched[i], singles[i]), i
+
+
+def test_caller_workspace_equals_default(fmm, orc, torch):
+    say("caller workspace")
+    n, p = 1 << 15, 8
+    x = orc.normal(n, 6)
+    plan = fmm.Plan(n, p, 16, 32, fmm.C64)
+    xd = to_dev(torch, x, plan.torch_dtype)
+    ws = torch.empty(plan.workspace_bytes(1), dtype=torch.uint8, device="cuda")
+    a = plan.execute(xd)
+    b = plan.execute_ws(xd, torch.empty_like(xd), ws)
+    torch.cuda.synchronize()
+    assert torch.equal(a, b)
+
+
+def test_invalid_arguments_raise(fmm, torch):
+    plan = fmm.Plan(1 << 14, 4, 16, 32, fmm.C128)
+    x = torch.zeros(1 << 14, dtype=torch.complex64, device="cuda")
+    with pytest.raises(ValueError):
+        plan.execute(x)
+    with pytest.raises(fmm.FmmFftError):
+        fmm.Plan(1 << 14, 3, 16, 32, fmm.C128)
+
+
+# ---------------------------------------------------------------------------
+# full benchmark sizes
+# ---------------------------------------------------------------------------
+
+def test_fp64_plain_form_full_size(fmm, orc, torch):
+    """N=2^24 complex128, P=32 (M=2^19), p=20, leaf 64, N(0,1) input."""
+    say("fp64 full size N=2^24")
+    n = 1 << 24
+    x = orc.normal(n, 2)
+    _, X = run_plan(fmm, torch, x, n, 32, 20, 64, fmm.C128)
+    err = rel(X, orc.fft64(x))
+    say(f"  rel-L2 {err:.3e}")
+    assert err < 1e-11
+
+
+def tones_input(n, seed):
+    """16 tones at random integer bins, amplitude 10^U(-4,0), random phase,
+    plus complex N(0, sigma=1e-3) noise."""
+    rng = np.random.default_rng(seed)
+    k = np.arange(n)
+    x = (rng.standard_normal(n) + 1j * rng.standard_normal(n)) * 1e-3
+    for _ in range(16):
+        f = rng.integers(0, n)
+        amp = 10 ** rng.uniform(-4, 0)
+        ph = rng.uniform(0, 2 * np.pi)
+        x += amp * np.exp(1j * (2 * np.pi * ((f * k) % n) / n + ph))
+    return x
+
+
+def test_fp32_precision_form_full_size(fmm, orc, torch):
+    """N=2^26 complex64, P=64 (M=2^20), p=16, leaf 64, 16 tones + noise."""
+    say("fp32 full size N=2^26")
+    n = 1 << 26
+    x = tones_input(n, 3).astype(np.complex64).astype(np.complex128)
+    _, X = run_plan(fmm, torch, x, n, 64, 16, 64, fmm.C64)
+    Y = orc.fft64(x)
+    err = rel(X, Y)
+    say(f"  rel-L2 {err:.3e} max-abs/max {np.max(np.abs(X - Y)) / np.max(np.abs(Y)):.3e}")
+    assert err < 1e-6
+
+
+def test_fp32_batched_form(fmm, orc, torch):
+    """256 signals of N=2^20 complex64, P=16, p=12, leaf 32; per-signal scale
+    10^U(-3,3); rel-L2 <= 1e-6 per signal (16 signals checked on the CPU)."""
+    say("fp32 batched 256 x 2^20")
+    n, count = 1 << 20, 256
+    rng = np.random.default_rng(4)
+    plan = fmm.Plan(n, 16, 12, 32, fmm.C64, max_batch=count)
+    scales = 10 ** rng.uniform(-3, 3, count)
+    xd = torch.empty(count, n, dtype=torch.complex64, device="cuda")
+    gen = torch.Generator(device="cuda").manual_seed(4)
+    xd.real.normal_(generator=gen)
+    xd.imag.normal_(generator=gen)
+    xd *= torch.from_numpy(scales).to(torch.float32).cuda()[:, None]
+    X = plan.execute(xd)
+    torch.cuda.synchronize()
+    for i in rng.choice(count, 16, replace=False):
+        xi = xd[i].cpu().numpy().astype(np.complex128)
+        err = rel(X[i].cpu().numpy().astype(np.complex128), orc.fft64(xi))
+        assert err < 1e-6, (i, err)
