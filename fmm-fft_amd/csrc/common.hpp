@@ -53,7 +53,7 @@ struct Geom {
     int s, n;       // leaf boxes per section (2^n), tree depth
 };
 
-// Complex offset of level l (levels 2..n, 2^l nodes of [S][2][terms] reals).
+// Real offset of level l (levels 2..n, 2^l nodes of [terms][S][2] reals).
 __host__ __device__ inline int64_t level_off(const Geom &g, int l) {
     return ((int64_t(1) << l) - 4) * int64_t(g.S) * 2 * g.terms;
 }

@@ -55,7 +55,7 @@ int validate(int64_t n, int32_t P, int32_t terms, int32_t leaf, int32_t dtype, P
     if (!is_pow2(s) || s < 8) return FMMFFT_EINVAL;
     if (leaf % 4) return FMMFFT_ENOTSUP;                   // leaf lanes own 4 targets
     if ((int64_t)leaf * P > (dtype == FMMFFT_C64 ? 8192 : 4096)) return FMMFFT_ENOTSUP;
-    if (!is_pow2(M) || M > (int64_t(1) << 24)) return FMMFFT_ENOTSUP;
+    if (!is_pow2(M) || M > (int64_t(1) << 20)) return FMMFFT_ENOTSUP;  // two FFT passes of <= 1024
     Geom &g = out->g;
     g.N = n;
     g.M = M;
@@ -88,7 +88,7 @@ struct FftLayout {
 FftLayout fft_layout(const Geom &g) {
     FftLayout f{};
     const int m = ilog2(g.M);
-    if (g.M <= 4096) {
+    if (g.M <= 1024) {
         f.two_pass = false;
         f.M1 = g.M;
         f.M2 = 1;
@@ -116,8 +116,8 @@ TableLayout table_layout(const Geom &g, const FftLayout &f) {
     o = (o + 1) & ~size_t(1);
     t.gfac = o; o += 2 * (size_t)g.P;
     t.wP = o; o += 2 * (size_t)g.P;
-    t.twA = o; o += 2 * (size_t)std::max<int64_t>(1, f.M1 / 2);
-    t.twB = o; o += 2 * (size_t)std::max<int64_t>(1, f.M2 / 2);
+    t.twA = o; o += 2 * (size_t)f.M1;  // full circle: e^{-2 pi i k / M1}, k < M1
+    t.twB = o; o += 2 * (size_t)f.M2;
     const int64_t R = int64_t(1) << f.logR;
     t.tlo = o; o += 2 * (size_t)R;
     t.thi = o; o += 2 * (size_t)std::max<int64_t>(1, g.M / R);
@@ -131,23 +131,28 @@ struct Shapes {
     size_t s2m_lds;
     int leaf_G, leaf_bb;
     size_t leaf_lds;
-    int comb_rb;
-    size_t comb_lds;
-    int fftA_C, fftB_C;
-    size_t fftA_lds, fftB_lds;
+    bool leaf_mma;  // matrix-core leaf kernel (leaf size a multiple of 16)
 };
+
+// Register/LDS padding of the Chebyshev order for the row kernels.
+inline int padded_terms(int terms) { return terms <= 8 ? 8 : terms <= 16 ? 16 : terms <= 24 ? 24 : 32; }
 
 size_t leaf_lds(const Geom &g, size_t esize, int G, int bb) {
     return (size_t)(bb + 3) * g.b * G * 2 * esize + (size_t)G * (4 * g.b + 1) * esize +
            (size_t)(bb + 1) * G * 2 * g.terms * esize;
 }
 
-Shapes shapes(const Geom &g, size_t esize, const FftLayout &f) {
+size_t leaf_mma_lds(const Geom &g, size_t esize, int G) {
+    return (size_t)11 * g.b * G * 2 * esize + (size_t)G * 4 * g.b * esize;
+}
+
+Shapes shapes(const Geom &g, size_t esize) {
     Shapes sh{};
     const size_t lds_cap = 64 * 1024;
-    sh.s2m_bpb = std::max(1, g.s / 2048);
+    const int per = 256 / g.P;  // boxes in flight per S2M workgroup
+    sh.s2m_bpb = per * std::max(1, g.s / (per * 2048));
     sh.s2m_nblk = (g.s + sh.s2m_bpb - 1) / sh.s2m_bpb;
-    sh.s2m_lds = (size_t)g.b * g.P * 2 * esize;
+    sh.s2m_lds = (size_t)g.b * padded_terms(g.terms) * esize;
 
     int G = std::min(g.P, esize == 4 ? 16 : 8);
     int bb = std::min(g.s, 4);
@@ -156,20 +161,17 @@ Shapes shapes(const Geom &g, size_t esize, const FftLayout &f) {
     sh.leaf_G = G;
     sh.leaf_bb = bb;
     sh.leaf_lds = leaf_lds(g, esize, G, bb);
-
-    sh.comb_rb = (int)std::min<int64_t>(g.M, std::max(1, 2048 / g.P));
-    sh.comb_lds = (size_t)sh.comb_rb * (g.P + 1) * 16;
-
-    const int64_t budget = esize == 4 ? 4096 : 2048;  // complex elements in LDS
-    if (!f.two_pass) {
-        sh.fftA_C = (int)std::min<int64_t>(g.P, std::max<int64_t>(1, budget / f.M1));
-        sh.fftA_lds = (size_t)sh.fftA_C * (f.M1 + 1) * 2 * esize;
-    } else {
-        sh.fftA_C = (int)std::min<int64_t>({16, f.M2, std::max<int64_t>(1, budget / f.M1)});
-        sh.fftA_lds = (size_t)sh.fftA_C * (f.M1 + 1) * 2 * esize;
-        sh.fftB_C = (int)std::min<int64_t>({16, f.M1, std::max<int64_t>(1, budget / f.M2)});
-        sh.fftB_lds = (size_t)sh.fftB_C * (f.M2 + 1) * 2 * esize;
+    sh.leaf_mma = g.b % 16 == 0 && g.b <= 128;  // b is a power of two: b/16 in {1,2,4,8}
+    if (sh.leaf_mma) {
+        // one 64-byte x segment per window row, 8 boxes per workgroup
+        int Gm = std::min(g.P, esize == 4 ? 8 : 4);
+        while (leaf_mma_lds(g, esize, Gm) > lds_cap && Gm > 1) Gm >>= 1;
+        sh.leaf_G = Gm;
+        sh.leaf_bb = 8;
+        sh.leaf_lds = leaf_mma_lds(g, esize, Gm);
     }
+
+
     return sh;
 }
 
@@ -235,6 +237,78 @@ TabPtrs<float> tab_ptrs<float>(const fmmfft_plan_s *p) {
 
 inline unsigned blocks_for(int64_t n, int per = 256) { return (unsigned)((n + per - 1) / per); }
 
+template <typename T, int L>
+void fft_len(const Geom &g, const FftPass &fp, unsigned nz, hipStream_t st, const cx<T> *in, cx<T> *out,
+             int64_t in_stride, int64_t out_stride, const cx<T> *tw, const c64 *tlo, const c64 *thi) {
+    constexpr int C = fft_cols<T, L>();
+    const unsigned nwg = (unsigned)((fp.F * g.P + C - 1) / C);
+    hipLaunchKernelGGL((k_fft<T, L>), dim3(nwg, 1, nz), dim3(256), 0, st, g, fp, in, out, in_stride, out_stride, tw,
+                       tlo, thi);
+}
+
+template <typename T>
+void launch_fft(int64_t L, const Geom &g, const FftPass &fp, unsigned nz, hipStream_t st, const cx<T> *in,
+                cx<T> *out, int64_t in_stride, int64_t out_stride, const cx<T> *tw, const c64 *tlo,
+                const c64 *thi) {
+    switch (L) {
+        case 16: fft_len<T, 16>(g, fp, nz, st, in, out, in_stride, out_stride, tw, tlo, thi); break;
+        case 32: fft_len<T, 32>(g, fp, nz, st, in, out, in_stride, out_stride, tw, tlo, thi); break;
+        case 64: fft_len<T, 64>(g, fp, nz, st, in, out, in_stride, out_stride, tw, tlo, thi); break;
+        case 128: fft_len<T, 128>(g, fp, nz, st, in, out, in_stride, out_stride, tw, tlo, thi); break;
+        case 256: fft_len<T, 256>(g, fp, nz, st, in, out, in_stride, out_stride, tw, tlo, thi); break;
+        case 512: fft_len<T, 512>(g, fp, nz, st, in, out, in_stride, out_stride, tw, tlo, thi); break;
+        default: fft_len<T, 1024>(g, fp, nz, st, in, out, in_stride, out_stride, tw, tlo, thi); break;
+    }
+}
+
+template <typename T>
+void launch_combine(const Geom &g, unsigned nz, hipStream_t st, const cx<T> *psec, int64_t ps, const c64 *sumsec,
+                    int64_t ss, const c64 *gfac, const c64 *wP, cx<T> *y, int64_t ys) {
+    const dim3 grid((unsigned)((g.M + 63) / 64), 1, nz);
+#define COMBINE(PP) hipLaunchKernelGGL((k_combine<T, PP>), grid, dim3(256), 0, st, g, psec, ps, sumsec, ss, gfac, wP, y, ys)
+    switch (g.P) {
+        case 2: COMBINE(2); break;
+        case 4: COMBINE(4); break;
+        case 8: COMBINE(8); break;
+        case 16: COMBINE(16); break;
+        case 32: COMBINE(32); break;
+        default: COMBINE(64); break;
+    }
+#undef COMBINE
+}
+
+template <typename T, int PT, typename Mark>
+void s2m_tree_pt(const Geom &g, const Shapes &sh, unsigned nz, hipStream_t st, const cx<T> *x, T *phi, T *psi,
+                 int64_t wstride_T, c64 *sums_part, int64_t wstride_c64, c64 *sumsec, const TabPtrs<T> &tb,
+                 int conj, Mark &mark) {
+    hipLaunchKernelGGL((k_s2m<T, PT>), dim3(sh.s2m_nblk, 1, nz), dim3(256), sh.s2m_lds, st, g, x, g.N, phi,
+                       wstride_T, sums_part, wstride_c64, tb.initch, sh.s2m_bpb, conj);
+    hipLaunchKernelGGL(k_sums_reduce, dim3(g.P, 1, nz), dim3(256), 0, st, g, sums_part, sh.s2m_nblk, wstride_c64,
+                       sumsec, wstride_c64);
+    mark(1);
+    const int64_t rows = 2 * (int64_t)g.S;
+    for (int l = g.n - 1; l >= 2; --l)
+        hipLaunchKernelGGL((k_tree<T, PT>), dim3(blocks_for((int64_t(1) << l) * rows), 1, nz), dim3(256), 0, st, g,
+                           l, 0, (const T *)phi, phi, (const T *)psi, psi, wstride_T, tb.ops, tb.topflp);
+    mark(2);
+    for (int l = 2; l <= g.n; ++l)
+        hipLaunchKernelGGL((k_tree<T, PT>), dim3(blocks_for((int64_t(1) << l) * rows), 1, nz), dim3(256), 0, st, g,
+                           l, l == 2 ? 2 : 1, (const T *)phi, phi, (const T *)psi, psi, wstride_T, tb.ops,
+                           tb.topflp);
+}
+
+template <typename T, typename Mark>
+void launch_s2m_tree(int pt, const fmmfft_plan_s *, const Geom &g, const Shapes &sh, unsigned nz, hipStream_t st,
+                     const cx<T> *x, T *phi, T *psi, int64_t wstride_T, c64 *sums_part, int64_t wstride_c64,
+                     c64 *sumsec, const TabPtrs<T> &tb, int conj, Mark &mark) {
+    switch (pt) {
+        case 8: s2m_tree_pt<T, 8>(g, sh, nz, st, x, phi, psi, wstride_T, sums_part, wstride_c64, sumsec, tb, conj, mark); break;
+        case 16: s2m_tree_pt<T, 16>(g, sh, nz, st, x, phi, psi, wstride_T, sums_part, wstride_c64, sumsec, tb, conj, mark); break;
+        case 24: s2m_tree_pt<T, 24>(g, sh, nz, st, x, phi, psi, wstride_T, sums_part, wstride_c64, sumsec, tb, conj, mark); break;
+        default: s2m_tree_pt<T, 32>(g, sh, nz, st, x, phi, psi, wstride_T, sums_part, wstride_c64, sumsec, tb, conj, mark); break;
+    }
+}
+
 template <typename T>
 int run(const fmmfft_plan_s *p, int64_t count, const void *d_in, void *d_out, int dir, char *ws,
         hipStream_t st) {
@@ -269,69 +343,58 @@ int run(const fmmfft_plan_s *p, int64_t count, const void *d_in, void *d_out, in
         if (ev) (void)hipEventRecord(ev[i], st);
     };
     mark(0);
-    hipLaunchKernelGGL(k_s2m<T>, dim3(sh.s2m_nblk, 1, nz), dim3(256), sh.s2m_lds, st, g, x, g.N, phi,
-                       wstride_T, sums_part, wstride_c64, tb.initch, sh.s2m_bpb, conj);
-    hipLaunchKernelGGL(k_sums_reduce, dim3(g.P, 1, nz), dim3(256), 0, st, g, sums_part, sh.s2m_nblk,
-                       wstride_c64, sumsec, wstride_c64);
-    mark(1);
-    const int64_t node_len = (int64_t)g.S * 2 * g.terms;
-    for (int l = g.n - 1; l >= 2; --l)
-        hipLaunchKernelGGL(k_m2m<T>, dim3(blocks_for((int64_t(1) << l) * node_len), 1, nz), dim3(256), 0, st,
-                           g, l, phi, wstride_T, tb.ops);
-    mark(2);
-    for (int l = 2; l <= g.n; ++l)
-        hipLaunchKernelGGL(k_down<T>, dim3(blocks_for((int64_t(1) << l) * node_len), 1, nz), dim3(256), 0, st,
-                           g, l, phi, psi, wstride_T, tb.ops, tb.topflp);
+    launch_s2m_tree<T>(padded_terms(g.terms), p, g, sh, nz, st, x, phi, psi, wstride_T, sums_part, wstride_c64,
+                       sumsec, tb, conj, mark);
     mark(3);
-    hipLaunchKernelGGL(k_leaf<T>, dim3(g.s / sh.leaf_bb, g.P / sh.leaf_G, nz), dim3(256), sh.leaf_lds, st, g,
-                       x, g.N, psi, wstride_T, psec, wstride_cx, tb.evalm, tb.evalmh, tb.kv, sh.leaf_G,
-                       sh.leaf_bb, conj);
+    if (sh.leaf_mma) {
+        const unsigned nwg = (unsigned)((g.P / sh.leaf_G) * (g.s / 8));
+#define LEAF_MMA(NRT)                                                                                        \
+    hipLaunchKernelGGL((k_leaf_mma<T, NRT>), dim3(nwg, 1, nz), dim3(256), sh.leaf_lds, st, g, x, g.N, psi, \
+                       wstride_T, psec, wstride_cx, tb.evalm, tb.evalmh, tb.kv, sh.leaf_G, conj)
+        switch (g.b / 16) {
+            case 1: LEAF_MMA(1); break;
+            case 2: LEAF_MMA(2); break;
+            case 4: LEAF_MMA(4); break;
+            default: LEAF_MMA(8); break;
+        }
+#undef LEAF_MMA
+    } else {
+        hipLaunchKernelGGL(k_leaf<T>, dim3(g.s / sh.leaf_bb, g.P / sh.leaf_G, nz), dim3(256), sh.leaf_lds, st, g,
+                           x, g.N, psi, wstride_T, psec, wstride_cx, tb.evalm, tb.evalmh, tb.kv, sh.leaf_G,
+                           sh.leaf_bb, conj);
+    }
     mark(4);
-    hipLaunchKernelGGL(k_combine<T>, dim3((unsigned)(g.M / sh.comb_rb), 1, nz), dim3(256), sh.comb_lds, st, g,
-                       psec, wstride_cx, sumsec, wstride_c64, gfac, wP, out, g.N, sh.comb_rb);
+    launch_combine<T>(g, nz, st, psec, wstride_cx, sumsec, wstride_c64, gfac, wP, out, g.N);
     mark(5);
 
     const FftLayout &f = p->fft;
+    const cx<T> *twA = reinterpret_cast<const cx<T> *>(tb.twA);
+    const cx<T> *twB = reinterpret_cast<const cx<T> *>(tb.twB);
     if (!f.two_pass) {
         FftPass fp{};
-        fp.L = (int)f.M1;
-        fp.logL = ilog2(f.M1);
-        fp.C = sh.fftA_C;
         fp.F = 1;
         fp.in_fs = g.M; fp.in_es = 1;
         fp.out_fs = g.M; fp.out_ks = 1;
-        fp.twiddle = 0;
         fp.conj_out = conj;
         fp.logR = f.logR;
-        hipLaunchKernelGGL(k_fft<T>, dim3((unsigned)(g.P / fp.C), 1, nz), dim3(256), sh.fftA_lds, st, g, fp,
-                           out, out, g.N, g.N, tb.twA, tlo, thi);
+        launch_fft<T>(f.M1, g, fp, nz, st, out, out, g.N, g.N, twA, tlo, thi);
         mark(6);
     } else {
         FftPass a{};
-        a.L = (int)f.M1;
-        a.logL = ilog2(f.M1);
-        a.C = sh.fftA_C;
         a.F = f.M2;
         a.in_fs = 1; a.in_es = f.M2;
         a.out_fs = 1; a.out_ks = f.M2;
         a.twiddle = 1;
-        a.conj_out = 0;
         a.logR = f.logR;
-        hipLaunchKernelGGL(k_fft<T>, dim3((unsigned)(g.P * f.M2 / a.C), 1, nz), dim3(256), sh.fftA_lds, st, g, a,
-                           out, psec, g.N, wstride_cx, tb.twA, tlo, thi);
+        launch_fft<T>(f.M1, g, a, nz, st, out, psec, g.N, wstride_cx, twA, tlo, thi);
         mark(6);
         FftPass bq{};
-        bq.L = (int)f.M2;
-        bq.logL = ilog2(f.M2);
-        bq.C = sh.fftB_C;
         bq.F = f.M1;
         bq.in_fs = f.M2; bq.in_es = 1;
         bq.out_fs = 1; bq.out_ks = f.M1;
-        bq.twiddle = 0;
         bq.conj_out = conj;
         bq.logR = f.logR;
-        hipLaunchKernelGGL(k_fft<T>, dim3((unsigned)(g.P * f.M1 / bq.C), 1, nz), dim3(256), sh.fftB_lds, st, g,
-                           bq, psec, out, wstride_cx, g.N, tb.twB, tlo, thi);
+        launch_fft<T>(f.M2, g, bq, nz, st, psec, out, wstride_cx, g.N, twB, tlo, thi);
     }
     mark(7);
     hipError_t e = hipGetLastError();
@@ -357,11 +420,11 @@ int fmmfft_plan_sizes(int64_t n, int32_t p_blocks, int32_t terms, int32_t leaf, 
     if (rc != FMMFFT_OK) return rc;
     const FftLayout f = fft_layout(prm.g);
     const TableLayout tl = table_layout(prm.g, f);
-    const Shapes sh = shapes(prm.g, prm.esize, f);
+    const Shapes sh = shapes(prm.g, prm.esize);
     const WsLayout wl = ws_layout(prm.g, prm.esize, sh);
     if (table_bytes) {
         size_t t = tl.total * sizeof(double);
-        if (dtype == FMMFFT_C64) t += (tl.n_ops + f.M1 + f.M2 + 4) * sizeof(float);
+        if (dtype == FMMFFT_C64) t += (tl.n_ops + 2 * f.M1 + 2 * f.M2) * sizeof(float);
         *table_bytes = t;
     }
     if (work_bytes) *work_bytes = wl.stride;
@@ -381,7 +444,7 @@ int fmmfft_plan_create(fmmfft_plan *out, int64_t n, int32_t p_blocks, int32_t te
     p->prm = prm;
     p->fft = fft_layout(prm.g);
     p->tl = table_layout(prm.g, p->fft);
-    p->sh = shapes(prm.g, prm.esize, p->fft);
+    p->sh = shapes(prm.g, prm.esize);
     p->ws = ws_layout(prm.g, prm.esize, p->sh);
     p->max_batch = max_batch == 0 ? 1 : max_batch;
     const Geom &g = prm.g;
@@ -413,16 +476,16 @@ int fmmfft_plan_create(fmmfft_plan *out, int64_t n, int32_t p_blocks, int32_t te
     hipLaunchKernelGGL(k_build_interp, dim3(jobs), dim3(64), 0, st, g, t + tl.ops, t + tl.topflp,
                        t + tl.evalm, t + tl.evalmh);
     const int64_t R = int64_t(1) << p->fft.logR;
-    const int64_t MA = p->fft.M1, MB = std::max<int64_t>(2, p->fft.M2);
-    hipLaunchKernelGGL(k_build_twiddles, dim3(blocks_for(MA / 2 + R + g.M / R)), dim3(256), 0, st, MA, g.M, R,
+    const int64_t MA = p->fft.M1, MB = p->fft.M2;
+    hipLaunchKernelGGL(k_build_twiddles, dim3(blocks_for(MA + R + g.M / R)), dim3(256), 0, st, MA, g.M, R,
                        reinterpret_cast<double2 *>(t + tl.twA), reinterpret_cast<double2 *>(t + tl.tlo),
                        reinterpret_cast<double2 *>(t + tl.thi));
     if (p->fft.two_pass)
-        hipLaunchKernelGGL(k_build_twiddles, dim3(blocks_for(MB / 2)), dim3(256), 0, st, MB, g.M, int64_t(0),
+        hipLaunchKernelGGL(k_build_twiddles, dim3(blocks_for(MB)), dim3(256), 0, st, MB, g.M, int64_t(0),
                            reinterpret_cast<double2 *>(t + tl.twB), reinterpret_cast<double2 *>(t + tl.tlo),
                            reinterpret_cast<double2 *>(t + tl.thi));
     if (prm.dtype == FMMFFT_C64) {
-        const size_t nA = 2 * (size_t)std::max<int64_t>(1, MA / 2), nB = 2 * (size_t)std::max<int64_t>(1, MB / 2);
+        const size_t nA = 2 * (size_t)MA, nB = 2 * (size_t)MB;
         p->tw32_A = tl.n_ops;
         p->tw32_B = tl.n_ops + nA;
         const size_t n32 = tl.n_ops + nA + nB;

@@ -1,42 +1,183 @@
 // Length-M local FFTs of the FMM-FFT (the reference's fftw_execute,
 // src/fmmfft.c:1080) on gfx950.
 //
-// M <= 4096: one pass, one LDS-resident FFT per block row.  Larger M: the
+// M <= 1024: one pass, one LDS-resident FFT per block row.  Larger M: the
 // four-step split M = M1 * M2, mu = M2*m1 + m2, k = k1 + M1*k2:
 //   pass 1: Z[k1][m2] = w_M^{m2 k1} * sum_m1 y[M2 m1 + m2] w_M1^{m1 k1}
 //   pass 2: X[k1 + M1 k2] = sum_m2 Z[k1][m2] w_M2^{m2 k2}
-// Each workgroup holds C sub-FFTs of length L in LDS (C consecutive columns,
-// so every global access is C contiguous elements), runs an in-place radix-2
-// DIT on bit-reversed input and writes the C results.  Pass-1 twiddles are
-// formed and applied in FP64 from two short tables (w_M^e = hi[e>>r] lo[e&R-1]).
+// A workgroup holds C sub-FFTs of length L in LDS; they are C consecutive
+// columns (or rows), so every global access moves C (or L) contiguous
+// elements.  Inside LDS the transform is a Stockham autosort FFT with radix
+// 16/8/4 butterflies held in registers (three LDS round trips for L = 1024).
+// Pass-1 twiddles are formed and applied in FP64 from two short tables
+// (w_M^e = hi[e >> r] * lo[e & (R-1)]).
 #pragma once
 #include "common.hpp"
 
 namespace fmmfft {
 
 struct FftPass {
-    int L, logL, C;       // sub-FFT length, log2, sub-FFTs per workgroup
-    int64_t F;            // sub-FFTs per length-M block
-    int64_t in_fs, in_es; // element e of sub-FFT f at k2*M + f*in_fs + e*in_es
+    int64_t F;             // sub-FFTs per length-M block
+    int64_t in_fs, in_es;  // element e of sub-FFT f at k2*M + f*in_fs + e*in_es
     int64_t out_fs, out_ks;
-    int twiddle;          // 1: multiply output k of sub-FFT f by w_M^{f k}
-    int conj_out;         // conjugate the output (inverse transform)
-    int logR;             // split of the pass-1 twiddle exponent
+    int twiddle;           // 1: multiply output k of sub-FFT f by w_M^{f k}
+    int conj_out;          // conjugate the output (inverse transform)
+    int logR;              // split of the pass-1 twiddle exponent
 };
 
 template <typename T>
+__device__ inline cx<T> cadd(cx<T> a, cx<T> b) { return cx<T>{a.re + b.re, a.im + b.im}; }
+template <typename T>
+__device__ inline cx<T> csub(cx<T> a, cx<T> b) { return cx<T>{a.re - b.re, a.im - b.im}; }
+// multiply by -i
+template <typename T>
+__device__ inline cx<T> cmi(cx<T> a) { return cx<T>{a.im, -a.re}; }
+
+// Forward radix-R DFTs in registers: y_q = sum_r x_r e^{-2 pi i r q / R}.
+template <typename T>
+__device__ inline void dft2(cx<T> &a, cx<T> &b) {
+    cx<T> t = a;
+    a = cadd(t, b);
+    b = csub(t, b);
+}
+
+template <typename T>
+__device__ inline void dft4(cx<T> *v) {
+    cx<T> s02 = cadd(v[0], v[2]), d02 = csub(v[0], v[2]);
+    cx<T> s13 = cadd(v[1], v[3]), d13 = cmi(csub(v[1], v[3]));
+    v[0] = cadd(s02, s13);
+    v[2] = csub(s02, s13);
+    v[1] = cadd(d02, d13);
+    v[3] = csub(d02, d13);
+}
+
+template <typename T>
+__device__ inline void dft8(cx<T> *v) {
+    const T h = T(0.70710678118654752440084436210484903928);
+    cx<T> e[4] = {v[0], v[2], v[4], v[6]}, o[4] = {v[1], v[3], v[5], v[7]};
+    dft4(e);
+    dft4(o);
+    // w8^1 = (1 - i) h, w8^2 = -i, w8^3 = -(1 + i) h
+    cx<T> o1{(o[1].re + o[1].im) * h, (o[1].im - o[1].re) * h};
+    cx<T> o2 = cmi(o[2]);
+    cx<T> o3{(o[3].im - o[3].re) * h, -(o[3].re + o[3].im) * h};
+    v[0] = cadd(e[0], o[0]); v[4] = csub(e[0], o[0]);
+    v[1] = cadd(e[1], o1);   v[5] = csub(e[1], o1);
+    v[2] = cadd(e[2], o2);   v[6] = csub(e[2], o2);
+    v[3] = cadd(e[3], o3);   v[7] = csub(e[3], o3);
+}
+
+template <typename T>
+__device__ inline void dft16(cx<T> *v) {
+    // w16^q = cos(pi q / 8) - i sin(pi q / 8)
+    const T c1 = T(0.92387953251128675612818318939678828682), s1 = T(0.38268343236508977172845998403039886676);
+    const T h = T(0.70710678118654752440084436210484903928);
+    cx<T> e[8], o[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        e[i] = v[2 * i];
+        o[i] = v[2 * i + 1];
+    }
+    dft8(e);
+    dft8(o);
+    const cx<T> w[8] = {{T(1), T(0)}, {c1, -s1}, {h, -h}, {s1, -c1}, {T(0), T(-1)}, {-s1, -c1}, {-h, -h}, {-c1, -s1}};
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        cx<T> t = q == 0 ? o[0] : cmulT(o[q], w[q]);
+        v[q] = cadd(e[q], t);
+        v[q + 8] = csub(e[q], t);
+    }
+}
+
+template <typename T, int R>
+__device__ inline void dftR(cx<T> *v) {
+    if constexpr (R == 2) dft2(v[0], v[1]);
+    else if constexpr (R == 4) dft4(v);
+    else if constexpr (R == 8) dft8(v);
+    else dft16(v);
+}
+
+constexpr int pick_radix(int rem) {
+    return rem == 2 ? 2 : rem == 4 ? 4 : rem == 8 ? 8 : rem == 32 ? 8 : rem == 512 ? 8 : 16;
+}
+
+// LDS index of element e of a row: one pad slot per 16 elements breaks the
+// bank aliasing of the radix-16 stride.
+__device__ inline int lpad(int e) { return e + (e >> 4); }
+
+// One Stockham stage (current sub-transform size Ns, radix R) over C rows.
+template <typename T, int L, int C, int Ns, int R>
+__device__ inline void stockham_stage(cx<T> *a, int ls, const cx<T> *__restrict__ twL) {
+    constexpr int NB = L / R;
+    constexpr int TOT = C * NB;
+    constexpr int BPT = (TOT + 255) / 256;
+    cx<T> v[BPT][R];
+#pragma unroll
+    for (int bq = 0; bq < BPT; ++bq) {
+        const int w = threadIdx.x + bq * 256;
+        if (TOT % 256 == 0 || w < TOT) {
+            const int f = w / NB, j = w % NB;
+            const cx<T> *row = a + f * ls;
+            const int k = j & (Ns - 1);
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                cx<T> x = row[lpad(j + r * NB)];
+                if (Ns > 1 && r > 0) x = cmulT(x, twL[r * k * (L / (Ns * R))]);
+                v[bq][r] = x;
+            }
+            dftR<T, R>(v[bq]);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int bq = 0; bq < BPT; ++bq) {
+        const int w = threadIdx.x + bq * 256;
+        if (TOT % 256 == 0 || w < TOT) {
+            const int f = w / NB, j = w % NB;
+            cx<T> *row = a + f * ls;
+            const int k = j & (Ns - 1);
+            const int d = (j / Ns) * Ns * R + k;
+#pragma unroll
+            for (int r = 0; r < R; ++r) row[lpad(d + r * Ns)] = v[bq][r];
+        }
+    }
+    __syncthreads();
+}
+
+template <typename T, int L, int C, int Ns>
+__device__ inline void stockham(cx<T> *a, int ls, const cx<T> *__restrict__ twL) {
+    if constexpr (Ns < L) {
+        constexpr int R = pick_radix(L / Ns);
+        stockham_stage<T, L, C, Ns, R>(a, ls, twL);
+        stockham<T, L, C, Ns * R>(a, ls, twL);
+    }
+}
+
+template <int L>
+constexpr int row_stride() { return L + L / 16 + 1; }
+
+// Sub-FFTs per workgroup: 8192 (complex64) or 4096 (complex128) elements,
+// at most 32; about 70 KB of LDS.
+template <typename T, int L>
+constexpr int fft_cols() {
+    constexpr int elems = sizeof(T) == 4 ? 8192 : 4096;
+    return elems / L > 32 ? 32 : elems / L;
+}
+
+// C sub-FFTs of length L per workgroup; twL = e^{-2 pi i k / L}, k < L.
+template <typename T, int L, int C = fft_cols<T, L>()>
 __global__ __launch_bounds__(256) void k_fft(Geom g, FftPass fp, const cx<T> *__restrict__ in,
                                              cx<T> *__restrict__ out, int64_t in_stride, int64_t out_stride,
-                                             const T *__restrict__ tw,  // [L/2] (re,im) w_L^k
-                                             const c64 *__restrict__ tlo, const c64 *__restrict__ thi) {
-    extern __shared__ __align__(16) unsigned char smem[];
-    const int L = fp.L, C = fp.C, logL = fp.logL;
-    const int ls = L + 1;  // padded row
-    cx<T> *a = reinterpret_cast<cx<T> *>(smem);  // [C][L+1]
+                                             const cx<T> *__restrict__ twL, const c64 *__restrict__ tlo,
+                                             const c64 *__restrict__ thi) {
+    constexpr int ls = row_stride<L>();
+    __shared__ cx<T> a[C * ls];
     const int sig = blockIdx.z;
     in += (int64_t)sig * in_stride;
     out += (int64_t)sig * out_stride;
     const int64_t fg0 = (int64_t)blockIdx.x * C;  // first global sub-FFT
+    const int64_t nf = fp.F * g.P;               // sub-FFTs per signal
+    const int cv = (int)min((int64_t)C, nf - fg0);  // valid sub-FFTs here
     const int64_t k2 = fg0 / fp.F;
     const int64_t f0 = fg0 - k2 * fp.F;
     const int64_t base = k2 * g.M;
@@ -44,35 +185,20 @@ __global__ __launch_bounds__(256) void k_fft(Geom g, FftPass fp, const cx<T> *__
     const int64_t fstep_in = fp.F == 1 ? g.M : fp.in_fs;
     const int64_t fstep_out = fp.F == 1 ? g.M : fp.out_fs;
 
-    const int n = C * L;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    constexpr int n = C * L;
+    for (int i = threadIdx.x; i < n; i += 256) {
         int f, e;
         if (fp.in_es == 1) { e = i % L; f = i / L; } else { f = i % C; e = i / C; }
-        const cx<T> v = in[base + (f0 + f) * fstep_in + (int64_t)e * fp.in_es];
-        const int br = (int)(__builtin_bitreverse32((unsigned)e) >> (32 - logL));
-        a[f * ls + br] = v;
+        if (f < cv) a[f * ls + lpad(e)] = in[base + (f0 + f) * fstep_in + (int64_t)e * fp.in_es];
     }
     __syncthreads();
-    const cx<T> *twc = reinterpret_cast<const cx<T> *>(tw);
-    for (int h = 1; h < L; h <<= 1) {
-        const int tstep = L / (2 * h);
-        for (int bf = threadIdx.x; bf < n / 2; bf += blockDim.x) {
-            const int f = bf / (L / 2), t = bf - f * (L / 2);
-            const int grp = t / h, k = t - grp * h;
-            const int i1 = f * ls + grp * 2 * h + k, i2 = i1 + h;
-            const cx<T> w = twc[k * tstep];
-            const cx<T> u = a[i1];
-            const cx<T> v = cmulT(a[i2], w);
-            a[i1] = cx<T>{u.re + v.re, u.im + v.im};
-            a[i2] = cx<T>{u.re - v.re, u.im - v.im};
-        }
-        __syncthreads();
-    }
+    stockham<T, L, C, 1>(a, ls, twL);
     const int64_t Rm = (int64_t(1) << fp.logR) - 1;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    for (int i = threadIdx.x; i < n; i += 256) {
         int f, k;
         if (fp.out_ks == 1) { k = i % L; f = i / L; } else { f = i % C; k = i / C; }
-        cx<T> v = a[f * ls + k];
+        if (f >= cv) continue;
+        cx<T> v = a[f * ls + lpad(k)];
         if (fp.twiddle) {
             const int64_t e = ((f0 + f) * (int64_t)k) & (g.M - 1);
             const c64 w = cmul(thi[e >> fp.logR], tlo[e & Rm]);

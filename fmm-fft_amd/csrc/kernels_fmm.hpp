@@ -6,13 +6,15 @@
 //   x     [M][P]  complex, natural order: section sc is column sc
 //                 (the reference's polyphase permute, src/fmmfft.c:419-439,
 //                 is only an index map here)
-//   tree  levels 2..n, level l has 2^l nodes of [P-1][2][terms] reals
-//                 (re and im planes of each section's expansion)
-//   psec  [M][P]  complex: column sc >= 1 is the cot sum of section sc,
-//                 column 0 is a copy of x_0
+//   tree  levels 2..n, level l has 2^l nodes of [terms][P-1][2] reals:
+//                 term-major, rows (section, re/im) innermost, so lanes that
+//                 own consecutive rows access consecutive addresses
+//   psec  [P][M]  complex: row sc >= 1 is the cot sum of section sc,
+//                 row 0 is a copy of x_0
 //   y     [P][M]  complex: block k2 feeds the k2-th length-M FFT
 #pragma once
 #include "common.hpp"
+#include "kernels_fft.hpp"
 
 namespace fmmfft {
 
@@ -21,56 +23,74 @@ namespace fmmfft {
 // The reference's initmm (src/fmmfft.c:199-211, called at :465):
 //   phi_leaf[box][sc][k] = sum_j x_sc[box*b + j] * initch[j][k]
 // and its per-rank section sums (src/fmmfft.c:441-450), accumulated in FP64.
-// One workgroup handles `bpb` consecutive leaf boxes; a box is one
-// contiguous [b][P] tile of x.
+// A lane owns one (box, section): it streams the section's b samples straight
+// from HBM (lanes of a wave read consecutive sections of one row: coalesced),
+// keeps 2*terms accumulators in registers and reads initch from LDS
+// (broadcast).  The workgroup covers 256/P boxes at a time and loops over
+// `bpb` boxes in total.
 // ---------------------------------------------------------------------------
-template <typename T>
+template <typename T, int PT>
 __global__ __launch_bounds__(256) void k_s2m(Geom g, const cx<T> *__restrict__ x, int64_t x_stride,
                                              T *__restrict__ phi, int64_t tree_stride,
                                              c64 *__restrict__ sums_part, int64_t sums_stride,
                                              const T *__restrict__ initch, int bpb, int conj) {
     extern __shared__ __align__(16) unsigned char smem[];
-    const int P = g.P, b = g.b, terms = g.terms, S = g.S;
-    cx<T> *tile = reinterpret_cast<cx<T> *>(smem);  // [b][P]
+    const int P = g.P, b = g.b, terms = g.terms;
+    const int rows = 2 * g.S;
+    T *ich = reinterpret_cast<T *>(smem);  // [b][PT], zero padded
+    __shared__ double red[2][256];
     const int sig = blockIdx.z;
     x += (int64_t)sig * x_stride;
     phi += (int64_t)sig * tree_stride + level_off(g, g.n);
     sums_part += (int64_t)sig * sums_stride + (int64_t)blockIdx.x * P;
+    for (int i = threadIdx.x; i < b * PT; i += blockDim.x) {
+        const int j = i / PT, k = i - j * PT;
+        ich[i] = k < terms ? initch[j * terms + k] : T(0);
+    }
+    __syncthreads();
 
-    double sre = 0.0, sim = 0.0;  // thread t < P: running sum of section t
-    const int box0 = blockIdx.x * bpb;
-    const int nout = S * 2 * terms;
-    for (int bx = 0; bx < bpb; ++bx) {
-        const int box = box0 + bx;
-        if (box >= g.s) break;
-        __syncthreads();
-        const cx<T> *src = x + (int64_t)box * b * P;
-        for (int i = threadIdx.x; i < b * P; i += blockDim.x) {
-            cx<T> v = src[i];
+    const int per = blockDim.x / P;  // boxes in flight
+    const int slot = threadIdx.x / P, sc = threadIdx.x - slot * P;
+    double sre = 0.0, sim = 0.0;
+    const int box_end = min(g.s, (blockIdx.x + 1) * bpb);
+    for (int box = blockIdx.x * bpb + slot; slot < per && box < box_end; box += per) {
+        const cx<T> *src = x + (int64_t)box * b * P + sc;
+        T are[PT], aim[PT];
+#pragma unroll
+        for (int k = 0; k < PT; ++k) are[k] = aim[k] = T(0);
+        for (int j = 0; j < b; ++j) {
+            cx<T> v = src[(int64_t)j * P];
             if (conj) v.im = -v.im;
-            tile[i] = v;
-        }
-        __syncthreads();
-        if (threadIdx.x < P) {
-            for (int j = 0; j < b; ++j) {
-                cx<T> v = tile[j * P + threadIdx.x];
-                sre += (double)v.re;
-                sim += (double)v.im;
+            sre += (double)v.re;
+            sim += (double)v.im;
+            const T *ir = ich + j * PT;
+#pragma unroll
+            for (int k = 0; k < PT; ++k) {
+                are[k] += v.re * ir[k];
+                aim[k] += v.im * ir[k];
             }
         }
-        T *dst = phi + (int64_t)box * nout;
-        for (int o = threadIdx.x; o < nout; o += blockDim.x) {
-            const int row = o / terms, k = o - row * terms;
-            const int sc = (row >> 1) + 1, c = row & 1;
-            T acc = 0;
-            for (int j = 0; j < b; ++j) {
-                const cx<T> v = tile[j * P + sc];
-                acc += (c ? v.im : v.re) * initch[j * terms + k];
-            }
-            dst[o] = acc;
+        if (sc > 0) {
+            T *dst = phi + (int64_t)box * terms * rows + 2 * (sc - 1);
+#pragma unroll
+            for (int k = 0; k < PT; ++k)
+                if (k < terms) {
+                    dst[(int64_t)k * rows] = are[k];
+                    dst[(int64_t)k * rows + 1] = aim[k];
+                }
         }
     }
-    if (threadIdx.x < P) sums_part[threadIdx.x] = c64{sre, sim};
+    red[0][threadIdx.x] = sre;
+    red[1][threadIdx.x] = sim;
+    __syncthreads();
+    if (threadIdx.x < P) {
+        double re = 0.0, im = 0.0;
+        for (int s2 = 0; s2 < per; ++s2) {
+            re += red[0][s2 * P + threadIdx.x];
+            im += red[1][s2 * P + threadIdx.x];
+        }
+        sums_part[threadIdx.x] = c64{re, im};
+    }
 }
 
 // sumsec[sc] = (i/M) * sum over blocks of the partial sums (the reference's
@@ -106,90 +126,130 @@ __global__ __launch_bounds__(256) void k_sums_reduce(Geom g, const c64 *__restri
 }
 
 // ---------------------------------------------------------------------------
-// Upward pass, parent level l: phi_l[q] = shftfn . phi_{l+1}[2q] +
-// shftfp . phi_{l+1}[2q+1] (src/fmmfft.c:476-515, mpp :174-184).
-// One lane per output (node, section, re/im, term).
+// One tree level, one lane per output row (node, section, re/im): the lane
+// forms all `terms` outputs as a sum of up to four p x p operators applied to
+// input rows, with the operators staged in LDS (every lane of a wave reads
+// the same entry: broadcast) and the rows read term by term (lanes read
+// consecutive rows: coalesced).
+//   mode 0, upward, parent level l (src/fmmfft.c:476-515, mpp :174-184):
+//     phi_l[q] = shftfn . phi_{l+1}[2q] + shftfp . phi_{l+1}[2q+1]
+//   mode 1, downward, level l >= 3 (src/fmmfft.c:644-749):
+//     even c: shftlp . psi_{l-1}[c/2] + flip2p phi_l[c-2] + flip2n phi_l[c+2] + flip3n phi_l[c+3]
+//     odd  c: shftln . psi_{l-1}[c/2] + flip3p phi_l[c-3] + flip2p phi_l[c-2] + flip2n phi_l[c+2]
+//     node indices periodic (the reference's ring of ranks)
+//   mode 2, top flip (src/fmmfft.c:620-639): psi_2[q] = topflp . phi_2[q+2 mod 4]
 // ---------------------------------------------------------------------------
-template <typename T>
-__global__ __launch_bounds__(256) void k_m2m(Geom g, int l, T *__restrict__ phi, int64_t tree_stride,
-                                             const T *__restrict__ ops) {
-    const int terms = g.terms, rows = g.S * 2, node_len = rows * terms;
-    const int64_t nout = (int64_t(1) << l) * node_len;
-    const int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (o >= nout) return;
-    phi += (int64_t)blockIdx.z * tree_stride;
-    const int64_t q = o / node_len;
-    const int r = (int)(o - q * node_len);
-    const int row = r / terms, k = r - row * terms;
-    const T *A = ops + ((size_t)(l - 2) * NOPS + OP_SHFTFN) * terms * terms + (size_t)k * terms;
-    const T *B = ops + ((size_t)(l - 2) * NOPS + OP_SHFTFP) * terms * terms + (size_t)k * terms;
-    const T *cl = phi + level_off(g, l + 1) + (2 * q) * node_len + (int64_t)row * terms;
-    const T *cr = cl + node_len;
-    T acc = 0;
-    for (int kk = 0; kk < terms; ++kk) acc += A[kk] * cl[kk] + B[kk] * cr[kk];
-    phi[level_off(g, l) + o] = acc;
-}
-
-// ---------------------------------------------------------------------------
-// Downward pass at level l.  l == 2: psi_2[q] = topflp . phi_2[q+2 mod 4]
-// (src/fmmfft.c:620-639).  l >= 3: psi_l[c] = L2L(psi_{l-1}[c/2]) plus three
-// M2L flips from the interaction list (src/fmmfft.c:644-749):
-//   even c: shftlp, flip2p(c-2), flip2n(c+2), flip3n(c+3)
-//   odd  c: shftln, flip3p(c-3), flip2p(c-2), flip2n(c+2)
-// with periodic node indices (the reference's ring of ranks).
-// ---------------------------------------------------------------------------
-template <typename T>
-__global__ __launch_bounds__(256) void k_down(Geom g, int l, const T *__restrict__ phi, T *__restrict__ psi,
-                                              int64_t tree_stride, const T *__restrict__ ops,
-                                              const T *__restrict__ topflp) {
-    const int terms = g.terms, rows = g.S * 2, node_len = rows * terms;
-    const int nn = 1 << l;
-    const int64_t nout = (int64_t)nn * node_len;
-    const int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (o >= nout) return;
-    phi += (int64_t)blockIdx.z * tree_stride;
-    psi += (int64_t)blockIdx.z * tree_stride;
-    const int c = (int)(o / node_len);
-    const int r = (int)(o - (int64_t)c * node_len);
-    const int row = r / terms, k = r - row * terms;
-    const T *ph = phi + level_off(g, l) + (int64_t)row * terms;
-    const int tt = terms * terms;
-    T acc = 0;
-    if (l == 2) {
-        const T *A = topflp + (size_t)k * terms;
-        const T *src = ph + (int64_t)((c + 2) & 3) * node_len;
-        for (int kk = 0; kk < terms; ++kk) acc += A[kk] * src[kk];
+template <typename T, int PT>
+__global__ __launch_bounds__(256) void k_tree(Geom g, int l, int mode, const T *__restrict__ phi_in,
+                                              T *__restrict__ phi_out, const T *__restrict__ psi_in,
+                                              T *__restrict__ psi_out, int64_t tree_stride,
+                                              const T *__restrict__ ops, const T *__restrict__ topflp) {
+    __shared__ T As[6 * PT * PT];
+    const int terms = g.terms, rows = 2 * g.S, tt = terms * terms;
+    const int node_len = rows * terms;
+    // stage the operators, zero padded to PT x PT
+    int nmat;
+    const T *mats[6];
+    if (mode == 0) {
+        const T *o = ops + (size_t)(l - 2) * NOPS * tt;
+        nmat = 2;
+        mats[0] = o + OP_SHFTFN * tt;
+        mats[1] = o + OP_SHFTFP * tt;
+    } else if (mode == 1) {
+        const T *o = ops + (size_t)(l - 3) * NOPS * tt;
+        nmat = 6;
+        mats[0] = o + OP_SHFTLP * tt;
+        mats[1] = o + OP_SHFTLN * tt;
+        mats[2] = o + OP_FLIP2P * tt;
+        mats[3] = o + OP_FLIP2N * tt;
+        mats[4] = o + OP_FLIP3N * tt;
+        mats[5] = o + OP_FLIP3P * tt;
     } else {
-        const T *o8 = ops + (size_t)(l - 3) * NOPS * tt + (size_t)k * terms;
-        const T *par = psi + level_off(g, l - 1) + (int64_t)(c >> 1) * node_len + (int64_t)row * terms;
-        const T *Ls, *F0, *F1, *F2;
-        const T *s0, *s1, *s2;
-        if ((c & 1) == 0) {
-            Ls = o8 + OP_SHFTLP * tt;
-            F0 = o8 + OP_FLIP2P * tt; s0 = ph + (int64_t)pmod(c - 2, nn) * node_len;
-            F1 = o8 + OP_FLIP2N * tt; s1 = ph + (int64_t)pmod(c + 2, nn) * node_len;
-            F2 = o8 + OP_FLIP3N * tt; s2 = ph + (int64_t)pmod(c + 3, nn) * node_len;
-        } else {
-            Ls = o8 + OP_SHFTLN * tt;
-            F0 = o8 + OP_FLIP3P * tt; s0 = ph + (int64_t)pmod(c - 3, nn) * node_len;
-            F1 = o8 + OP_FLIP2P * tt; s1 = ph + (int64_t)pmod(c - 2, nn) * node_len;
-            F2 = o8 + OP_FLIP2N * tt; s2 = ph + (int64_t)pmod(c + 2, nn) * node_len;
-        }
-        for (int kk = 0; kk < terms; ++kk)
-            acc += Ls[kk] * par[kk] + F0[kk] * s0[kk] + F1[kk] * s1[kk] + F2[kk] * s2[kk];
+        nmat = 1;
+        mats[0] = topflp;
     }
-    psi[level_off(g, l) + o] = acc;
+    for (int i = threadIdx.x; i < nmat * PT * PT; i += blockDim.x) {
+        const int m = i / (PT * PT), r = i - m * PT * PT;
+        const int k = r / PT, kk = r - k * PT;
+        const T *src = m == 0 ? mats[0] : m == 1 ? mats[1] : m == 2 ? mats[2] : m == 3 ? mats[3] : m == 4 ? mats[4] : mats[5];
+        As[i] = (k < terms && kk < terms) ? src[k * terms + kk] : T(0);
+    }
+    __syncthreads();
+
+    const int nn = 1 << l;
+    const int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (o >= (int64_t)nn * rows) return;
+    const int c = (int)(o / rows), row = (int)(o - (int64_t)c * rows);
+    const int64_t ts = (int64_t)blockIdx.z * tree_stride;
+
+    const T *src[4];
+    int mi[4];
+    int nt;
+    T *dst;
+    if (mode == 0) {
+        const T *ch = phi_in + ts + level_off(g, l + 1) + row;
+        src[0] = ch + (int64_t)(2 * c) * node_len;
+        src[1] = ch + (int64_t)(2 * c + 1) * node_len;
+        mi[0] = 0; mi[1] = 1;
+        nt = 2;
+        dst = phi_out + ts + level_off(g, l) + (int64_t)c * node_len + row;
+    } else if (mode == 1) {
+        const T *ph = phi_in + ts + level_off(g, l) + row;
+        src[0] = psi_in + ts + level_off(g, l - 1) + (int64_t)(c >> 1) * node_len + row;
+        if ((c & 1) == 0) {
+            mi[0] = 0;
+            mi[1] = 2; src[1] = ph + (int64_t)pmod(c - 2, nn) * node_len;
+            mi[2] = 3; src[2] = ph + (int64_t)pmod(c + 2, nn) * node_len;
+            mi[3] = 4; src[3] = ph + (int64_t)pmod(c + 3, nn) * node_len;
+        } else {
+            mi[0] = 1;
+            mi[1] = 5; src[1] = ph + (int64_t)pmod(c - 3, nn) * node_len;
+            mi[2] = 2; src[2] = ph + (int64_t)pmod(c - 2, nn) * node_len;
+            mi[3] = 3; src[3] = ph + (int64_t)pmod(c + 2, nn) * node_len;
+        }
+        nt = 4;
+        dst = psi_out + ts + level_off(g, l) + (int64_t)c * node_len + row;
+    } else {
+        src[0] = phi_in + ts + level_off(g, 2) + (int64_t)((c + 2) & 3) * node_len + row;
+        mi[0] = 0;
+        nt = 1;
+        dst = psi_out + ts + level_off(g, 2) + (int64_t)c * node_len + row;
+    }
+
+    T acc[PT];
+#pragma unroll
+    for (int k = 0; k < PT; ++k) acc[k] = T(0);
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        if (t < nt) {
+            T v[PT];
+#pragma unroll
+            for (int kk = 0; kk < PT; ++kk) v[kk] = kk < terms ? src[t][(int64_t)kk * rows] : T(0);
+            const T *A = As + mi[t] * PT * PT;
+#pragma unroll
+            for (int k = 0; k < PT; ++k) {
+                T a = acc[k];
+#pragma unroll
+                for (int kk = 0; kk < PT; ++kk) a += A[k * PT + kk] * v[kk];
+                acc[k] = a;
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < PT; ++k)
+        if (k < terms) dst[(int64_t)k * rows] = acc[k];
 }
 
 // ---------------------------------------------------------------------------
-// Leaf evaluation: psec[mu][sc] = L2T(local expansion of the home box) +
+// Leaf evaluation: psec[sc][mu] = L2T(local expansion of the home box) +
 // sum over the 3b near sources of K_sc(nu - mu) x_sc[nu],
 // K_sc(d) = -cot(pi (d + sc/P) / M) / M  (src/fmmfft.c:751-954).
 // Target mu = i*b + j of section sc sits at mu - sc/P; for j = 0 and
 // sc > P/2 its home is box i-1 (row evalmh, sources in boxes i-2..i), for
 // sc = P/2 both evaluations are averaged (src/fmmfft.c:771-803, 839-954).
-// Column 0 receives x_0 for the combine.
+// Row 0 receives x_0 for the combine.
 //
+// Vector-FMA form, used when the leaf size is not a multiple of 16.
 // Workgroup: `bb` consecutive boxes x `G` consecutive sections.  The x window
 // (boxes i0-2 .. i0+bb) of those sections, their kernel generators and the
 // local expansions of boxes i0-1 .. i0+bb-1 are staged in LDS.  A lane owns
@@ -238,7 +298,8 @@ __global__ __launch_bounds__(256) void k_leaf(Geom g, const cx<T> *__restrict__ 
         const int gg = rem / nl, t = rem - gg * nl;
         const int sc = sc0 + gg;
         const int box = pmod(i0 - 1 + bx, g.s);
-        loc[e] = sc == 0 ? T(0) : psi[((int64_t)box * S + (sc - 1)) * nl + t];
+        const int cc = t / terms, k = t - cc * terms;
+        loc[e] = sc == 0 ? T(0) : psi[((int64_t)box * terms + k) * (2 * S) + 2 * (sc - 1) + cc];
     }
     __syncthreads();
 
@@ -252,7 +313,7 @@ __global__ __launch_bounds__(256) void k_leaf(Geom g, const cx<T> *__restrict__ 
         const int j0 = 4 * jq;
         const int64_t mu0 = (int64_t)(i0 + bl) * b + j0;
         if (sc == 0) {
-            for (int r = 0; r < 4; ++r) psec[(mu0 + r) * P] = xw[((bl + 2) * b + j0 + r) * G + gg];
+            for (int r = 0; r < 4; ++r) psec[mu0 + r] = xw[((bl + 2) * b + j0 + r) * G + gg];
             continue;
         }
         const T *kr = kvs + gg * kvl;
@@ -307,59 +368,200 @@ __global__ __launch_bounds__(256) void k_leaf(Geom g, const cx<T> *__restrict__ 
                 aim[0] = him;
             }
         }
-        for (int r = 0; r < 4; ++r) psec[(mu0 + r) * P + sc] = cx<T>{are[r], aim[r]};
+        for (int r = 0; r < 4; ++r) psec[(int64_t)sc * M + mu0 + r] = cx<T>{are[r], aim[r]};
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Leaf evaluation on the matrix cores (leaf size b a multiple of 16).
+//
+// Same sums as k_leaf.  For one section the near field of 8 consecutive
+// boxes is the product
+//   D[j][(box, c)] = sum_{q < 3b} K_sc(q - b - j) X[q][(box, c)],
+//   X[q][(box, c)] = component c of x_sc[(i0 + box - 1) b + q]
+// of the b x 3b Toeplitz block (generated on the fly from the 4b kernel
+// samples in LDS) with the 3b x 16 window matrix (8 boxes x re/im), i.e.
+// b/16 row tiles of 16x16 MFMA outputs, K = 3b in steps of 4.  The L2T term
+// appends K = terms steps with the evaluation rows as A and the boxes' local
+// expansions as B.  Target 0 of a section with sc >= P/2 is re-homed to box
+// i-1 by adding the difference of the two evaluations (two b-long kernel dot
+// products and two L2T rows), in full for sc > P/2 and halved for sc = P/2.
+//
+// Workgroup: 4 waves, 8 boxes x G sections (one section per wave at a time).
+// LDS: the x window of boxes i0-2 .. i0+8 for the G sections and their
+// kernel samples.  The workgroup index is remapped so that workgroups that
+// share x lines (all section groups of one box chunk) run on one XCD.
+// MFMA lane layout (16x16x4): lane l supplies A[l%16][l/16] and B[l/16][l%16]
+// and receives D[4*(l/16) + r][l%16], r = 0..3.
+// ---------------------------------------------------------------------------
+template <typename T>
+struct Mfma;
+template <>
+struct Mfma<float> {
+    typedef float v4 __attribute__((ext_vector_type(4)));
+    static __device__ inline v4 mma(float a, float b, v4 c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+};
+template <>
+struct Mfma<double> {
+    typedef double v4 __attribute__((ext_vector_type(4)));
+    static __device__ inline v4 mma(double a, double b, v4 c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+};
+
+template <typename T, int NRT>
+__global__ __launch_bounds__(256) void k_leaf_mma(Geom g, const cx<T> *__restrict__ x, int64_t x_stride,
+                                                  const T *__restrict__ psi, int64_t tree_stride,
+                                                  cx<T> *__restrict__ psec, int64_t psec_stride,
+                                                  const T *__restrict__ evalm, const T *__restrict__ evalmh,
+                                                  const T *__restrict__ kv, int G, int conj) {
+    using V4 = typename Mfma<T>::v4;
+    constexpr int BB = 8;  // boxes per workgroup = MFMA columns / 2
+    extern __shared__ __align__(16) unsigned char smem[];
+    constexpr int b = 16 * NRT;
+    const int P = g.P, terms = g.terms, S = g.S;
+    const int64_t M = g.M;
+    const int rows = (BB + 3) * b;
+    const int rowsS = 2 * S;  // tree row stride
+    cx<T> *xw = reinterpret_cast<cx<T> *>(smem);           // [rows][G]
+    T *kvs = reinterpret_cast<T *>(xw + (size_t)rows * G);  // [G][4b]
+    const int sig = blockIdx.z;
+    x += (int64_t)sig * x_stride;
+    psi += (int64_t)sig * tree_stride + level_off(g, g.n);
+    psec += (int64_t)sig * psec_stride;
+
+    // XCD-aware order: consecutive logical indices share an XCD
+    const int ngroups = P / G;
+    const int nwg = gridDim.x * gridDim.y;
+    const int bid = blockIdx.y * gridDim.x + blockIdx.x;
+    const int lid = (nwg % 8 == 0) ? (bid % 8) * (nwg / 8) + bid / 8 : bid;
+    const int sc0 = (lid % ngroups) * G;
+    const int i0 = (lid / ngroups) * BB;
+
+    const int64_t mu_w0 = (int64_t)(i0 - 2) * b;
+    for (int e = threadIdx.x; e < rows * G; e += blockDim.x) {
+        const int r = e / G, gg = e - r * G;
+        int64_t mu = mu_w0 + r;
+        mu = mu < 0 ? mu + M : (mu >= M ? mu - M : mu);
+        cx<T> v = x[mu * P + sc0 + gg];
+        if (conj) v.im = -v.im;
+        xw[e] = v;
+    }
+    for (int e = threadIdx.x; e < G * 4 * b; e += blockDim.x) {
+        const int gg = e / (4 * b), d = e - gg * 4 * b;
+        const int sc = sc0 + gg;
+        kvs[e] = sc == 0 ? T(0) : kv[(size_t)(sc - 1) * 4 * b + d];
+    }
+    __syncthreads();
+
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int col = lane & 15, kr = lane >> 4;
+    const int bx = col >> 1, c = col & 1;
+    const T *xwc = reinterpret_cast<const T *>(xw);
+
+    for (int gg = wave; gg < G; gg += 4) {
+        const int sc = sc0 + gg;
+        if (sc == 0) {
+            cx<T> *dst = psec + (int64_t)i0 * b;
+            for (int t = lane; t < BB * b; t += 64) dst[t] = xw[(2 * b + t) * G];
+            continue;
+        }
+        const T *kr_ = kvs + gg * 4 * b;
+        V4 acc[NRT];
+#pragma unroll
+        for (int rt = 0; rt < NRT; ++rt) acc[rt] = V4{0, 0, 0, 0};
+        // near field: B row q = 4 s + kr of box bx's window (rows (bx+1) b + q)
+        const T *xb = xwc + ((size_t)(bx + 1) * b * G + gg) * 2 + c;
+        constexpr int ksteps = 3 * b / 4;
+#pragma unroll 4
+        for (int s = 0; s < ksteps; ++s) {
+            const int q = 4 * s + kr;
+            const T bv = xb[(size_t)q * G * 2];
+#pragma unroll
+            for (int rt = 0; rt < NRT; ++rt) {
+                const T av = kr_[q + b - 16 * rt - col];
+                acc[rt] = Mfma<T>::mma(av, bv, acc[rt]);
+            }
+        }
+        // far field (L2T): A = evalm rows, B = local expansion of box bx
+        const T *loc = psi + (int64_t)(pmod(i0 + bx, g.s)) * terms * rowsS + 2 * (sc - 1) + c;
+        const T *ev = evalm + (size_t)(sc - 1) * b * terms;
+        for (int s = 0; s < (terms + 3) / 4; ++s) {
+            const int k = 4 * s + kr;
+            const T bv = k < terms ? loc[(int64_t)k * rowsS] : T(0);
+#pragma unroll
+            for (int rt = 0; rt < NRT; ++rt) {
+                const T av = k < terms ? ev[(16 * rt + col) * terms + k] : T(0);
+                acc[rt] = Mfma<T>::mma(av, bv, acc[rt]);
+            }
+        }
+        if (2 * sc >= P) {
+            // re-home target 0 of box bx to box bx-1: lanes col, col+16, col+32,
+            // col+48 each take a quarter of the correction terms
+            T corr = 0;
+            const T *xm2 = xwc + ((size_t)bx * b * G + gg) * 2 + c;           // box i-2
+            const T *xp1 = xwc + ((size_t)(bx + 3) * b * G + gg) * 2 + c;     // box i+1
+            for (int t = kr; t < b; t += 4)
+                corr += kr_[t] * xm2[(size_t)t * G * 2] - kr_[3 * b + t] * xp1[(size_t)t * G * 2];
+            const T *locm = psi + (int64_t)(pmod(i0 + bx - 1, g.s)) * terms * rowsS + 2 * (sc - 1) + c;
+            const T *eh = evalmh + (size_t)(sc - P / 2) * terms;
+            for (int k = kr; k < terms; k += 4)
+                corr += eh[k] * locm[(int64_t)k * rowsS] - ev[k] * loc[(int64_t)k * rowsS];
+            corr += __shfl_xor(corr, 16, 64);
+            corr += __shfl_xor(corr, 32, 64);
+            if (kr == 0) acc[0][0] += (2 * sc == P) ? T(0.5) * corr : corr;
+        }
+        // write: lane holds targets 16 rt + 4 kr + r of box bx, component c
+        T *dst = reinterpret_cast<T *>(psec + (int64_t)sc * M + (int64_t)(i0 + bx) * b) + c;
+#pragma unroll
+        for (int rt = 0; rt < NRT; ++rt)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) dst[2 * (16 * rt + 4 * kr + r)] = acc[rt][r];
     }
 }
 
 // ---------------------------------------------------------------------------
 // Combine (src/fmmfft.c:956-970): for every row mu,
-//   y_k2[mu] = x_0[mu] + sum_sc w_P^{sc k2} gfac_sc (psec[mu][sc] - sumsec_sc),
-// i.e. the reference's dense fo product done as a P-point FFT, in FP64, and
-// written block-major (the reference's MPI_Alltoall).
+//   y_k2[mu] = x_0[mu] + sum_sc w_P^{sc k2} gfac_sc (psec[sc][mu] - sumsec_sc),
+// i.e. the reference's dense fo product done as a P-point FFT (Stockham in
+// LDS, FP64 arithmetic for both precisions) and written block-major (the
+// reference's MPI_Alltoall).  64 rows per workgroup; reads and writes are
+// 64 contiguous elements per section / block.
 // ---------------------------------------------------------------------------
-template <typename T>
+template <typename T, int P>
 __global__ __launch_bounds__(256) void k_combine(Geom g, const cx<T> *__restrict__ psec, int64_t psec_stride,
                                                  const c64 *__restrict__ sumsec, int64_t sumsec_stride,
                                                  const c64 *__restrict__ gfac, const c64 *__restrict__ wP,
-                                                 cx<T> *__restrict__ y, int64_t y_stride, int rb) {
-    extern __shared__ __align__(16) unsigned char smem[];
-    const int P = g.P, lp = __builtin_ctz(P);
-    const int rs = P + 1;  // padded row
-    c64 *a = reinterpret_cast<c64 *>(smem);  // [rb][P+1]
+                                                 cx<T> *__restrict__ y, int64_t y_stride) {
+    constexpr int RB = 64;
+    constexpr int ls = row_stride<P>();
+    __shared__ c64 a[RB * ls];
     const int sig = blockIdx.z;
     psec += (int64_t)sig * psec_stride;
     sumsec += (int64_t)sig * sumsec_stride;
     y += (int64_t)sig * y_stride;
-    const int64_t mu0 = (int64_t)blockIdx.x * rb;
+    const int64_t mu0 = (int64_t)blockIdx.x * RB;
+    const int rv = (int)min((int64_t)RB, g.M - mu0);
 
-    for (int e = threadIdx.x; e < rb * P; e += blockDim.x) {
-        const int r = e / P, sc = e - r * P;
-        const cx<T> v = psec[(mu0 + r) * P + sc];
+    for (int e = threadIdx.x; e < RB * P; e += 256) {
+        const int sc = e / RB, r = e - sc * RB;
+        if (r >= rv) continue;
+        const cx<T> v = psec[(int64_t)sc * g.M + mu0 + r];
         c64 w{(double)v.re, (double)v.im};
         if (sc != 0) {
             const c64 s = sumsec[sc];
             w = cmul(gfac[sc], c64{w.re - s.re, w.im - s.im});
         }
-        const int br = (int)(__builtin_bitreverse32((unsigned)sc) >> (32 - lp));
-        a[r * rs + (lp ? br : 0)] = w;
+        a[r * ls + lpad(sc)] = w;
     }
     __syncthreads();
-    for (int h = 1; h < P; h <<= 1) {
-        for (int bf = threadIdx.x; bf < rb * (P / 2); bf += blockDim.x) {
-            const int r = bf / (P / 2), t = bf - r * (P / 2);
-            const int grp = t / h, k = t - grp * h;
-            const int i1 = grp * 2 * h + k, i2 = i1 + h;
-            const c64 tw = wP[k * (P / (2 * h))];
-            const c64 u = a[r * rs + i1];
-            const c64 v = cmul(a[r * rs + i2], tw);
-            a[r * rs + i1] = c64{u.re + v.re, u.im + v.im};
-            a[r * rs + i2] = c64{u.re - v.re, u.im - v.im};
-        }
-        __syncthreads();
-    }
-    for (int e = threadIdx.x; e < rb * P; e += blockDim.x) {
-        const int k2 = e / rb, r = e - k2 * rb;
-        const c64 v = a[r * rs + k2];
+    stockham<double, P, RB, 1>(a, ls, wP);
+    for (int e = threadIdx.x; e < RB * P; e += 256) {
+        const int k2 = e / RB, r = e - k2 * RB;
+        if (r >= rv) continue;
+        const c64 v = a[r * ls + lpad(k2)];
         y[(int64_t)k2 * g.M + mu0 + r] = cx<T>{(T)v.re, (T)v.im};
     }
 }

@@ -186,19 +186,19 @@ __global__ __launch_bounds__(256) void k_build_closed(Geom g, double *cheb, doub
     }
 }
 
-// FFT twiddles: tw[k] = e^{-2 pi i k / L} for k < L/2 (stage twiddles of the
-// longest sub-FFT), lo[k] = e^{-2 pi i k / M} for k < R, hi[k] = e^{-2 pi i k R / M}
+// FFT twiddles: tw[k] = e^{-2 pi i k / L} for k < L (stage twiddles of a
+// sub-FFT), lo[k] = e^{-2 pi i k / M} for k < R, hi[k] = e^{-2 pi i k R / M}
 // for k < M/R (their product gives any e^{-2 pi i e / M}).
 __global__ __launch_bounds__(256) void k_build_twiddles(int64_t L, int64_t M, int64_t R, double2 *tw,
                                                          double2 *lo, double2 *hi) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     double s, c;
-    if (i < L / 2) {
+    if (i < L) {
         sincospi(2.0 * (double)i / (double)L, &s, &c);
         tw[i] = make_double2(c, -s);
         return;
     }
-    i -= L / 2;
+    i -= L;
     if (R <= 0) return;  // stage twiddles only
     if (i < R) {
         sincospi(2.0 * (double)i / (double)M, &s, &c);

@@ -114,8 +114,8 @@ int fmmfft_read_table(fmmfft_plan plan, int32_t which, void *host_dst, size_t by
  * default workspace (signal 0) to host memory; synchronises the device.
  * which: 0 section sums [P] complex128 ((i/M) * sum, entry 0 = 0),
  * 1 multipole tree, 2 local tree (both levels 2..depth, node-major,
- * [node][P-1][2][terms] real in the plan's precision),
- * 3 section cot sums psi [M][P] (column 0 holds x_0) in the plan's complex
+ * [node][terms][P-1][2] real in the plan's precision),
+ * 3 section cot sums psi [P][M] (row 0 holds x_0) in the plan's complex
  * type.  bytes must equal the buffer's size. */
 int fmmfft_read_stage(fmmfft_plan plan, int32_t which, void *host_dst, size_t bytes);
 

@@ -83,11 +83,11 @@ def test_stages_match_oracle_fp64(fmm, orc, torch, n, p, terms, leaf):
     psi = ref.downward(phi)
     tl = ref.tree_len()
     for which, expect in ((1, phi), (2, psi)):
-        raw = plan.read_stage(which, 2 * tl, np.float64).reshape(-1, S, 2, terms)
-        got = (raw[:, :, 0, :] + 1j * raw[:, :, 1, :]).reshape(-1)
+        raw = plan.read_stage(which, 2 * tl, np.float64).reshape(-1, terms, S, 2)
+        got = (raw[..., 0] + 1j * raw[..., 1]).transpose(0, 2, 1).reshape(-1)
         assert rel(got, expect) < 1e-13, which
     ps = ref.evaluate(x, psi).reshape(M, p)
-    g_ps = plan.read_stage(3, n, np.complex128).reshape(M, p)
+    g_ps = plan.read_stage(3, n, np.complex128).reshape(p, M).T
     assert rel(g_ps[:, 1:], ps[:, 1:]) < 1e-13
     assert np.array_equal(g_ps[:, 0], x.reshape(M, p)[:, 0])
 
