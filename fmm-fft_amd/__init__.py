@@ -43,11 +43,26 @@ EXPORTS = (
     "fmmfft_read_stage",
     "fmmfft_profile_begin",
     "fmmfft_profile_end",
+    "fmmfft_server_create",
+    "fmmfft_server_submit",
+    "fmmfft_server_wait",
+    "fmmfft_server_drain",
+    "fmmfft_server_stats_get",
+    "fmmfft_server_reset_stats",
+    "fmmfft_server_destroy",
     "fmmfft_strerror",
     "fmmfft_last_hip_error",
 )
 
 _lib = None
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int64) for n in ("completed", "batches", "plans_created", "plans_evicted",
+                                               "graphs_captured")] + \
+               [(n, ctypes.c_double) for n in ("p50_ms", "p99_ms", "mean_ms", "max_ms", "span_s",
+                                                "nominal_gflop")] + \
+               [(n, ctypes.c_size_t) for n in ("bytes_in_use", "peak_bytes")]
 
 
 class FmmFftError(RuntimeError):
@@ -79,6 +94,13 @@ def lib():
         L.fmmfft_read_stage.argtypes = [vp, i32, vp, sz]
         L.fmmfft_profile_begin.argtypes = [vp, i32]
         L.fmmfft_profile_end.argtypes = [vp, ctypes.POINTER(ctypes.c_double), i32, ctypes.POINTER(i64)]
+        L.fmmfft_server_create.argtypes = [ctypes.POINTER(vp), sz, i32]
+        L.fmmfft_server_submit.argtypes = [vp, i64, i32, i32, i32, i32, i32, vp, vp, ctypes.POINTER(ctypes.c_uint64)]
+        L.fmmfft_server_wait.argtypes = [vp, ctypes.c_uint64, ctypes.POINTER(ctypes.c_double)]
+        L.fmmfft_server_drain.argtypes = [vp]
+        L.fmmfft_server_stats_get.argtypes = [vp, ctypes.POINTER(ServerStats)]
+        L.fmmfft_server_reset_stats.argtypes = [vp]
+        L.fmmfft_server_destroy.argtypes = [vp]
         L.fmmfft_strerror.argtypes = [ctypes.c_int]
         L.fmmfft_strerror.restype = ctypes.c_char_p
         L.fmmfft_last_hip_error.restype = ctypes.c_int
@@ -223,3 +245,50 @@ def mfti(lq: int, p: int, terms: int, b: int, dtype: int = C128, max_batch: int 
 def mft(plan: Plan, qr, direction: int = FORWARD, stream=None):
     """In-place transform of all P blocks of ``qr`` (the reference's mft)."""
     return plan.execute(qr, qr, direction, stream)
+
+
+class Server:
+    """Request server (fmmfft_server_*): continuous batching of same-shape
+    requests under an HBM budget with cached plans and hipGraph replay.
+    Requests pass GPU tensors that must stay alive until wait() returns."""
+
+    def __init__(self, hbm_budget_bytes: int, max_batch: int = 64):
+        h = ctypes.c_void_p()
+        _check(lib().fmmfft_server_create(ctypes.byref(h), int(hbm_budget_bytes), int(max_batch)),
+               "fmmfft_server_create")
+        self._h = h
+
+    def submit(self, x, out, n: int, p_blocks: int, terms: int, leaf: int, dtype: int,
+               direction: int = FORWARD) -> int:
+        t = ctypes.c_uint64()
+        _check(lib().fmmfft_server_submit(self._h, n, p_blocks, terms, leaf, dtype, direction,
+                                          ctypes.c_void_p(x.data_ptr()), ctypes.c_void_p(out.data_ptr()),
+                                          ctypes.byref(t)), "fmmfft_server_submit")
+        return t.value
+
+    def wait(self, ticket: int) -> float:
+        ms = ctypes.c_double()
+        _check(lib().fmmfft_server_wait(self._h, ticket, ctypes.byref(ms)), "fmmfft_server_wait")
+        return ms.value
+
+    def drain(self):
+        _check(lib().fmmfft_server_drain(self._h), "fmmfft_server_drain")
+
+    def stats(self) -> dict:
+        st = ServerStats()
+        _check(lib().fmmfft_server_stats_get(self._h, ctypes.byref(st)), "fmmfft_server_stats_get")
+        return {n: getattr(st, n) for n, _ in ServerStats._fields_}
+
+    def reset_stats(self):
+        _check(lib().fmmfft_server_reset_stats(self._h), "fmmfft_server_reset_stats")
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().fmmfft_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

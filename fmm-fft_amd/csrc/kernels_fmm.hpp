@@ -392,16 +392,20 @@ __global__ __launch_bounds__(256) void k_leaf(Geom g, const cx<T> *__restrict__ 
 // kernel samples.  The workgroup index is remapped so that workgroups that
 // share x lines (all section groups of one box chunk) run on one XCD.
 // MFMA lane layout (16x16x4): lane l supplies A[l%16][l/16] and B[l/16][l%16]
-// and receives D[4*(l/16) + r][l%16], r = 0..3.
+// and receives D[row][l%16] for its four accumulators r = 0..3, with
+// row = 4*(l/16) + r for FP32 and l/16 + 4*r for FP64.
 // ---------------------------------------------------------------------------
 template <typename T>
 struct Mfma;
+// Output row of accumulator element r held by lanes with l/16 = kr (both
+// layouts measured on gfx950).
 template <>
 struct Mfma<float> {
     typedef float v4 __attribute__((ext_vector_type(4)));
     static __device__ inline v4 mma(float a, float b, v4 c) {
         return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
     }
+    static __device__ inline int row(int kr, int r) { return 4 * kr + r; }
 };
 template <>
 struct Mfma<double> {
@@ -409,6 +413,7 @@ struct Mfma<double> {
     static __device__ inline v4 mma(double a, double b, v4 c) {
         return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
     }
+    static __device__ inline int row(int kr, int r) { return kr + 4 * r; }
 };
 
 template <typename T, int NRT>
@@ -513,12 +518,12 @@ __global__ __launch_bounds__(256) void k_leaf_mma(Geom g, const cx<T> *__restric
             corr += __shfl_xor(corr, 32, 64);
             if (kr == 0) acc[0][0] += (2 * sc == P) ? T(0.5) * corr : corr;
         }
-        // write: lane holds targets 16 rt + 4 kr + r of box bx, component c
+        // write: lane holds targets 16 rt + row(kr, r) of box bx, component c
         T *dst = reinterpret_cast<T *>(psec + (int64_t)sc * M + (int64_t)(i0 + bx) * b) + c;
 #pragma unroll
         for (int rt = 0; rt < NRT; ++rt)
 #pragma unroll
-            for (int r = 0; r < 4; ++r) dst[2 * (16 * rt + 4 * kr + r)] = acc[rt][r];
+            for (int r = 0; r < 4; ++r) dst[2 * (16 * rt + Mfma<T>::row(kr, r))] = acc[rt][r];
     }
 }
 

@@ -131,6 +131,43 @@ int fmmfft_read_stage(fmmfft_plan plan, int32_t which, void *host_dst, size_t by
 int fmmfft_profile_begin(fmmfft_plan plan, int32_t max_calls);
 int fmmfft_profile_end(fmmfft_plan plan, double *stage_ms, int32_t nstages, int64_t *calls);
 
+/* ---------------------------------------------------------------------------
+ * Request server (no reference counterpart: the reference runs one transform
+ * per MPI job, src/fmmfft.c:1192-1288).  A scheduler thread admits requests
+ * of mixed (N, P, terms, leaf, dtype, dir) as they arrive, keeps plans and
+ * workspaces cached on the device under an HBM budget (evicting idle shapes,
+ * least recently used first), batches same-shape requests into one launch
+ * sequence replayed from a captured hipGraph (N <= 2^20; the requests are
+ * gathered into and scattered from a contiguous slot), runs larger shapes
+ * directly on the caller's buffers, and overlaps batches of different shapes
+ * on separate streams.  Requests carry caller-owned device buffers that must
+ * stay valid until fmmfft_server_wait returns; every result equals what
+ * fmmfft_execute returns for the same input.
+ * ------------------------------------------------------------------------- */
+typedef struct fmmfft_server_s *fmmfft_server;
+
+typedef struct fmmfft_server_stats {
+    int64_t completed, batches, plans_created, plans_evicted, graphs_captured;
+    double p50_ms, p99_ms, mean_ms, max_ms; /* submit -> completion latency  */
+    double span_s;                          /* first submit -> last completion */
+    double nominal_gflop;                   /* sum of 5 N log2 N, completed   */
+    size_t bytes_in_use, peak_bytes;        /* server-owned device memory    */
+} fmmfft_server_stats;
+
+int fmmfft_server_create(fmmfft_server *out, size_t hbm_budget_bytes, int32_t max_batch);
+/* Queue one transform; returns at once with a ticket.  EINVAL / ENOTSUP for
+ * parameters fmmfft_plan_create would reject; a shape that cannot fit the
+ * budget even alone completes with FMMFFT_ENOMEM. */
+int fmmfft_server_submit(fmmfft_server srv, int64_t n, int32_t p_blocks, int32_t terms, int32_t leaf,
+                         int32_t dtype, int32_t dir, const void *d_in, void *d_out, uint64_t *ticket);
+/* Block until the ticket's transform has finished on the device; returns its
+ * status and latency.  A ticket can be waited for once. */
+int fmmfft_server_wait(fmmfft_server srv, uint64_t ticket, double *latency_ms);
+int fmmfft_server_drain(fmmfft_server srv);
+int fmmfft_server_stats_get(fmmfft_server srv, fmmfft_server_stats *stats);
+int fmmfft_server_reset_stats(fmmfft_server srv);
+int fmmfft_server_destroy(fmmfft_server srv);
+
 const char *fmmfft_strerror(int status);
 /* The last failing HIP error code seen by the library (0 if none). */
 int fmmfft_last_hip_error(void);

@@ -1,0 +1,131 @@
+"""GPU tests of the request server (fmmfft_server_*, through the C ABI):
+every result of a mixed request stream equals the direct execute of the same
+input (bit for bit: batched and single launches run the same arithmetic),
+requests that cannot fit the HBM budget fail cleanly, idle shapes are
+evicted to admit new ones, and the statistics add up."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+
+
+@pytest.fixture(scope="module")
+def torch():
+    import torch
+
+    assert torch.cuda.is_available()
+    return torch
+
+
+def shape_for(k, terms, c128):
+    """Serving-form shape: N = 2^k, P = 2^min(6, k // 4), leaf 64 (32 when
+    M < 1024)."""
+    n = 1 << k
+    p = 1 << min(6, k // 4)
+    m = n // p
+    leaf = 64 if m >= 1024 else 32
+    return n, p, terms, leaf, 1 if c128 else 0
+
+
+def make_requests(torch, fmm, count, seed, kmin=12, kmax=17):
+    rng = np.random.default_rng(seed)
+    reqs = []
+    for i in range(count):
+        k = int(rng.integers(kmin, kmax + 1))
+        terms = int(rng.choice([12, 20]))
+        c128 = rng.random() < 0.3
+        n, p, t, leaf, dt = shape_for(k, terms, c128)
+        g = torch.Generator(device="cuda").manual_seed(int(seed * 100003 + i))
+        dtype = torch.complex128 if dt == 1 else torch.complex64
+        x = torch.randn(n, dtype=dtype, device="cuda", generator=g)
+        reqs.append(((n, p, t, leaf, dt), x, torch.empty_like(x)))
+    return reqs
+
+
+def test_mixed_stream_matches_direct_execute(fmm, orc, torch):
+    say("server mixed stream")
+    reqs = make_requests(torch, fmm, 48, 1)
+    srv = fmm.Server(8 << 30, max_batch=16)
+    tickets = [srv.submit(x, out, *shape) for shape, x, out in reqs]
+    lats = [srv.wait(t) for t in tickets]
+    st = srv.stats()
+    srv.close()
+    assert st["completed"] == len(reqs) and all(v >= 0 for v in lats)
+    assert st["p50_ms"] <= st["p99_ms"] <= st["max_ms"]
+    assert st["batches"] < len(reqs)  # same-shape requests were batched
+    plans = {}
+    for shape, x, out in reqs:
+        if shape not in plans:
+            plans[shape] = fmm.Plan(*shape)
+        ref = plans[shape].execute(x)
+        torch.cuda.synchronize()
+        assert torch.equal(out, ref), shape
+    # and against the FP64 FFT oracle for one request of each precision
+    for shape, x, out in reqs[:6]:
+        X = orc.fft64(x.cpu().numpy().astype(np.complex128))
+        err = np.linalg.norm(out.cpu().numpy() - X) / np.linalg.norm(X)
+        assert err < (1e-11 if shape[4] == 1 else 1e-6), (shape, err)
+
+
+def test_alone_equals_in_the_mix(fmm, torch):
+    say("server alone vs mix")
+    reqs = make_requests(torch, fmm, 24, 2)
+    srv = fmm.Server(8 << 30, max_batch=8)
+    alone = []
+    for shape, x, out in reqs[:4]:
+        o = torch.empty_like(x)
+        srv.wait(srv.submit(x, o, *shape))
+        alone.append(o)
+    tickets = [srv.submit(x, out, *shape) for shape, x, out in reqs]
+    for t in tickets:
+        srv.wait(t)
+    srv.close()
+    for a, (_, _, out) in zip(alone, reqs[:4]):
+        assert torch.equal(a, out)
+
+
+def test_request_larger_than_budget_fails_cleanly(fmm, torch):
+    say("server budget reject")
+    srv = fmm.Server(1 << 20, max_batch=4)  # 1 MiB: no plan fits
+    n, p, t, leaf, dt = shape_for(16, 12, False)
+    x = torch.zeros(n, dtype=torch.complex64, device="cuda")
+    out = torch.empty_like(x)
+    tk = srv.submit(x, out, n, p, t, leaf, dt)
+    with pytest.raises(fmm.FmmFftError) as e:
+        srv.wait(tk)
+    assert e.value.status == fmm.ENOMEM
+    with pytest.raises(fmm.FmmFftError):
+        srv.submit(x, out, n, 3, t, leaf, dt)  # invalid P
+    srv.close()
+
+
+def test_idle_shapes_are_evicted(fmm, torch):
+    say("server eviction")
+    a = shape_for(16, 12, False)
+    b = shape_for(16, 20, True)
+    _, wa = fmm.plan_sizes(*a)
+    _, wb = fmm.plan_sizes(*b)
+    # room for one shape (tables, two slots of workspace + gather buffers) at a time
+    ta, _ = fmm.plan_sizes(*a)
+    tb, _ = fmm.plan_sizes(*b)
+    na = ta + 2 * wa + 4 * a[0] * 8
+    nb = tb + 2 * wb + 4 * b[0] * 16
+    budget = max(na, nb) + min(na, nb) // 2
+    srv = fmm.Server(budget, max_batch=1)
+    outs = []
+    for i in range(6):
+        shape = a if i % 2 == 0 else b
+        dtype = torch.complex64 if shape[4] == 0 else torch.complex128
+        x = torch.randn(shape[0], dtype=dtype, device="cuda")
+        o = torch.empty_like(x)
+        srv.wait(srv.submit(x, o, *shape))
+        outs.append((shape, x, o))
+    st = srv.stats()
+    srv.close()
+    assert st["plans_evicted"] >= 1 and st["peak_bytes"] <= budget
+    for shape, x, o in outs:
+        assert torch.equal(o, fmm.Plan(*shape).execute(x))
