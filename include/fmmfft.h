@@ -116,7 +116,8 @@ int fmmfft_read_table(fmmfft_plan plan, int32_t which, void *host_dst, size_t by
  * 1 multipole tree, 2 local tree (both levels 2..depth, node-major,
  * [node][terms][P-1][2] real in the plan's precision),
  * 3 section cot sums psi [P][M] (row 0 holds x_0) in the plan's complex
- * type.  bytes must equal the buffer's size. */
+ * type -- only while M <= 1024: for larger M the two-pass local FFT reuses
+ * the buffer for its intermediate.  bytes must equal the buffer's size. */
 int fmmfft_read_stage(fmmfft_plan plan, int32_t which, void *host_dst, size_t bytes);
 
 /* Per-stage timing with HIP events recorded on each execute's own stream,
