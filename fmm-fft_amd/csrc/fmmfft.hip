@@ -264,7 +264,7 @@ void launch_fft(int64_t L, const Geom &g, const FftPass &fp, unsigned nz, hipStr
 template <typename T>
 void launch_combine(const Geom &g, unsigned nz, hipStream_t st, const cx<T> *psec, int64_t ps, const c64 *sumsec,
                     int64_t ss, const c64 *gfac, const c64 *wP, cx<T> *y, int64_t ys) {
-    const dim3 grid((unsigned)((g.M + 63) / 64), 1, nz);
+    const dim3 grid((unsigned)((g.M + 31) / 32), 1, nz);
 #define COMBINE(PP) hipLaunchKernelGGL((k_combine<T, PP>), grid, dim3(256), 0, st, g, psec, ps, sumsec, ss, gfac, wP, y, ys)
     switch (g.P) {
         case 2: COMBINE(2); break;

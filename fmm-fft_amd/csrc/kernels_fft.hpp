@@ -164,49 +164,77 @@ constexpr int fft_cols() {
     return elems / L > 32 ? 32 : elems / L;
 }
 
+// Workgroup index remap: the round-robin dispatch puts consecutive workgroup
+// ids on different XCDs (each with its own L2); remap so that each XCD runs a
+// contiguous range of logical ids, i.e. neighbouring column chunks that
+// share 128-byte lines are served by one L2.
+__device__ inline int64_t xcd_remap(int64_t bid, int64_t nwg) {
+    return (nwg % 8 == 0) ? (bid % 8) * (nwg / 8) + bid / 8 : bid;
+}
+
 // C sub-FFTs of length L per workgroup; twL = e^{-2 pi i k / L}, k < L.
+// Every lane first issues all of its global loads (PER elements, unrolled,
+// so they are in flight together), then fills LDS; the store side mirrors it.
 template <typename T, int L, int C = fft_cols<T, L>()>
 __global__ __launch_bounds__(256) void k_fft(Geom g, FftPass fp, const cx<T> *__restrict__ in,
                                              cx<T> *__restrict__ out, int64_t in_stride, int64_t out_stride,
                                              const cx<T> *__restrict__ twL, const c64 *__restrict__ tlo,
                                              const c64 *__restrict__ thi) {
     constexpr int ls = row_stride<L>();
+    constexpr int n = C * L;
+    constexpr int PER = n / 256;
+    static_assert(n % 256 == 0, "a workgroup holds a multiple of 256 elements");
     __shared__ cx<T> a[C * ls];
     const int sig = blockIdx.z;
     in += (int64_t)sig * in_stride;
     out += (int64_t)sig * out_stride;
-    const int64_t fg0 = (int64_t)blockIdx.x * C;  // first global sub-FFT
-    const int64_t nf = fp.F * g.P;               // sub-FFTs per signal
-    const int cv = (int)min((int64_t)C, nf - fg0);  // valid sub-FFTs here
+    const int64_t fg0 = xcd_remap(blockIdx.x, gridDim.x) * C;  // first global sub-FFT
+    const int64_t nf = fp.F * g.P;                            // sub-FFTs per signal
+    const int cv = (int)min((int64_t)C, nf - fg0);            // valid sub-FFTs here
     const int64_t k2 = fg0 / fp.F;
     const int64_t f0 = fg0 - k2 * fp.F;
     const int64_t base = k2 * g.M;
     // when F == 1 the C sub-FFTs are C consecutive blocks
     const int64_t fstep_in = fp.F == 1 ? g.M : fp.in_fs;
     const int64_t fstep_out = fp.F == 1 ? g.M : fp.out_fs;
+    const bool erow = fp.in_es == 1, krow = fp.out_ks == 1;
 
-    constexpr int n = C * L;
-    for (int i = threadIdx.x; i < n; i += 256) {
-        int f, e;
-        if (fp.in_es == 1) { e = i % L; f = i / L; } else { f = i % C; e = i / C; }
-        if (f < cv) a[f * ls + lpad(e)] = in[base + (f0 + f) * fstep_in + (int64_t)e * fp.in_es];
+    cx<T> v[PER];
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+        const int i = threadIdx.x + u * 256;
+        const int f = erow ? i / L : i % C, e = erow ? i % L : i / C;
+        if (f < cv) v[u] = in[base + (f0 + f) * fstep_in + (int64_t)e * fp.in_es];
+    }
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+        const int i = threadIdx.x + u * 256;
+        const int f = erow ? i / L : i % C, e = erow ? i % L : i / C;
+        a[f * ls + lpad(e)] = v[u];
     }
     __syncthreads();
     stockham<T, L, C, 1>(a, ls, twL);
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+        const int i = threadIdx.x + u * 256;
+        const int f = krow ? i / L : i % C, k = krow ? i % L : i / C;
+        v[u] = a[f * ls + lpad(k)];
+    }
     const int64_t Rm = (int64_t(1) << fp.logR) - 1;
-    for (int i = threadIdx.x; i < n; i += 256) {
-        int f, k;
-        if (fp.out_ks == 1) { k = i % L; f = i / L; } else { f = i % C; k = i / C; }
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+        const int i = threadIdx.x + u * 256;
+        const int f = krow ? i / L : i % C, k = krow ? i % L : i / C;
         if (f >= cv) continue;
-        cx<T> v = a[f * ls + lpad(k)];
+        cx<T> w = v[u];
         if (fp.twiddle) {
             const int64_t e = ((f0 + f) * (int64_t)k) & (g.M - 1);
-            const c64 w = cmul(thi[e >> fp.logR], tlo[e & Rm]);
-            const c64 r = cmul(c64{(double)v.re, (double)v.im}, w);
-            v = cx<T>{(T)r.re, (T)r.im};
+            const c64 t = cmul(thi[e >> fp.logR], tlo[e & Rm]);
+            const c64 r = cmul(c64{(double)w.re, (double)w.im}, t);
+            w = cx<T>{(T)r.re, (T)r.im};
         }
-        if (fp.conj_out) v.im = -v.im;
-        out[base + (f0 + f) * fstep_out + (int64_t)k * fp.out_ks] = v;
+        if (fp.conj_out) w.im = -w.im;
+        out[base + (f0 + f) * fstep_out + (int64_t)k * fp.out_ks] = w;
     }
 }
 

@@ -58,6 +58,7 @@ __global__ __launch_bounds__(256) void k_s2m(Geom g, const cx<T> *__restrict__ x
         T are[PT], aim[PT];
 #pragma unroll
         for (int k = 0; k < PT; ++k) are[k] = aim[k] = T(0);
+#pragma unroll 8
         for (int j = 0; j < b; ++j) {
             cx<T> v = src[(int64_t)j * P];
             if (conj) v.im = -v.im;
@@ -446,6 +447,7 @@ __global__ __launch_bounds__(256) void k_leaf_mma(Geom g, const cx<T> *__restric
     const int i0 = (lid / ngroups) * BB;
 
     const int64_t mu_w0 = (int64_t)(i0 - 2) * b;
+#pragma unroll 8
     for (int e = threadIdx.x; e < rows * G; e += blockDim.x) {
         const int r = e / G, gg = e - r * G;
         int64_t mu = mu_w0 + r;
@@ -454,6 +456,7 @@ __global__ __launch_bounds__(256) void k_leaf_mma(Geom g, const cx<T> *__restric
         if (conj) v.im = -v.im;
         xw[e] = v;
     }
+#pragma unroll 8
     for (int e = threadIdx.x; e < G * 4 * b; e += blockDim.x) {
         const int gg = e / (4 * b), d = e - gg * 4 * b;
         const int sc = sc0 + gg;
@@ -474,6 +477,21 @@ __global__ __launch_bounds__(256) void k_leaf_mma(Geom g, const cx<T> *__restric
             continue;
         }
         const T *kr_ = kvs + gg * 4 * b;
+        // issue the L2T operand loads (global) now; they land while the
+        // near-field MFMAs run
+        const T *loc = psi + (int64_t)(pmod(i0 + bx, g.s)) * terms * rowsS + 2 * (sc - 1) + c;
+        const T *ev = evalm + (size_t)(sc - 1) * b * terms;
+        constexpr int KT = 8;  // terms <= 32
+        const int kt = (terms + 3) / 4;
+        T la[NRT][KT], lb[KT];
+#pragma unroll
+        for (int s = 0; s < KT; ++s) {
+            const int k = 4 * s + kr;
+            const bool ok = s < kt && k < terms;
+            lb[s] = ok ? loc[(int64_t)k * rowsS] : T(0);
+#pragma unroll
+            for (int rt = 0; rt < NRT; ++rt) la[rt][s] = ok ? ev[(16 * rt + col) * terms + k] : T(0);
+        }
         V4 acc[NRT];
 #pragma unroll
         for (int rt = 0; rt < NRT; ++rt) acc[rt] = V4{0, 0, 0, 0};
@@ -491,17 +509,12 @@ __global__ __launch_bounds__(256) void k_leaf_mma(Geom g, const cx<T> *__restric
             }
         }
         // far field (L2T): A = evalm rows, B = local expansion of box bx
-        const T *loc = psi + (int64_t)(pmod(i0 + bx, g.s)) * terms * rowsS + 2 * (sc - 1) + c;
-        const T *ev = evalm + (size_t)(sc - 1) * b * terms;
-        for (int s = 0; s < (terms + 3) / 4; ++s) {
-            const int k = 4 * s + kr;
-            const T bv = k < terms ? loc[(int64_t)k * rowsS] : T(0);
 #pragma unroll
-            for (int rt = 0; rt < NRT; ++rt) {
-                const T av = k < terms ? ev[(16 * rt + col) * terms + k] : T(0);
-                acc[rt] = Mfma<T>::mma(av, bv, acc[rt]);
+        for (int s = 0; s < KT; ++s)
+            if (s < kt) {
+#pragma unroll
+                for (int rt = 0; rt < NRT; ++rt) acc[rt] = Mfma<T>::mma(la[rt][s], lb[s], acc[rt]);
             }
-        }
         if (2 * sc >= P) {
             // re-home target 0 of box bx to box bx-1: lanes col, col+16, col+32,
             // col+48 each take a quarter of the correction terms
@@ -532,16 +545,17 @@ __global__ __launch_bounds__(256) void k_leaf_mma(Geom g, const cx<T> *__restric
 //   y_k2[mu] = x_0[mu] + sum_sc w_P^{sc k2} gfac_sc (psec[sc][mu] - sumsec_sc),
 // i.e. the reference's dense fo product done as a P-point FFT (Stockham in
 // LDS, FP64 arithmetic for both precisions) and written block-major (the
-// reference's MPI_Alltoall).  64 rows per workgroup; reads and writes are
-// 64 contiguous elements per section / block.
+// reference's MPI_Alltoall).  32 rows per workgroup; reads and writes are
+// 32 contiguous elements per section / block, all issued before use.
 // ---------------------------------------------------------------------------
 template <typename T, int P>
 __global__ __launch_bounds__(256) void k_combine(Geom g, const cx<T> *__restrict__ psec, int64_t psec_stride,
                                                  const c64 *__restrict__ sumsec, int64_t sumsec_stride,
                                                  const c64 *__restrict__ gfac, const c64 *__restrict__ wP,
                                                  cx<T> *__restrict__ y, int64_t y_stride) {
-    constexpr int RB = 64;
+    constexpr int RB = 32;
     constexpr int ls = row_stride<P>();
+    constexpr int PER = (RB * P + 255) / 256;
     __shared__ c64 a[RB * ls];
     const int sig = blockIdx.z;
     psec += (int64_t)sig * psec_stride;
@@ -550,11 +564,19 @@ __global__ __launch_bounds__(256) void k_combine(Geom g, const cx<T> *__restrict
     const int64_t mu0 = (int64_t)blockIdx.x * RB;
     const int rv = (int)min((int64_t)RB, g.M - mu0);
 
-    for (int e = threadIdx.x; e < RB * P; e += 256) {
+    cx<T> v[PER];
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+        const int e = threadIdx.x + u * 256;
         const int sc = e / RB, r = e - sc * RB;
-        if (r >= rv) continue;
-        const cx<T> v = psec[(int64_t)sc * g.M + mu0 + r];
-        c64 w{(double)v.re, (double)v.im};
+        if (e < RB * P && r < rv) v[u] = psec[(int64_t)sc * g.M + mu0 + r];
+    }
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+        const int e = threadIdx.x + u * 256;
+        const int sc = e / RB, r = e - sc * RB;
+        if (e >= RB * P) continue;
+        c64 w{(double)v[u].re, (double)v[u].im};
         if (sc != 0) {
             const c64 s = sumsec[sc];
             w = cmul(gfac[sc], c64{w.re - s.re, w.im - s.im});
@@ -563,11 +585,13 @@ __global__ __launch_bounds__(256) void k_combine(Geom g, const cx<T> *__restrict
     }
     __syncthreads();
     stockham<double, P, RB, 1>(a, ls, wP);
-    for (int e = threadIdx.x; e < RB * P; e += 256) {
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+        const int e = threadIdx.x + u * 256;
         const int k2 = e / RB, r = e - k2 * RB;
-        if (r >= rv) continue;
-        const c64 v = a[r * ls + lpad(k2)];
-        y[(int64_t)k2 * g.M + mu0 + r] = cx<T>{(T)v.re, (T)v.im};
+        if (e >= RB * P || r >= rv) continue;
+        const c64 w = a[r * ls + lpad(k2)];
+        y[(int64_t)k2 * g.M + mu0 + r] = cx<T>{(T)w.re, (T)w.im};
     }
 }
 
