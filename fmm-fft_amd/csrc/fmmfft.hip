@@ -143,7 +143,7 @@ size_t leaf_lds(const Geom &g, size_t esize, int G, int bb) {
 }
 
 size_t leaf_mma_lds(const Geom &g, size_t esize, int G) {
-    return (size_t)11 * g.b * G * 2 * esize + (size_t)G * 4 * g.b * esize;
+    return ((size_t)11 * g.b * G + 11) * 2 * esize + (size_t)G * 4 * g.b * esize;
 }
 
 Shapes shapes(const Geom &g, size_t esize) {

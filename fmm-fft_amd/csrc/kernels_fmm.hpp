@@ -431,8 +431,8 @@ __global__ __launch_bounds__(256) void k_leaf_mma(Geom g, const cx<T> *__restric
     const int64_t M = g.M;
     const int rows = (BB + 3) * b;
     const int rowsS = 2 * S;  // tree row stride
-    cx<T> *xw = reinterpret_cast<cx<T> *>(smem);           // [rows][G]
-    T *kvs = reinterpret_cast<T *>(xw + (size_t)rows * G);  // [G][4b]
+    cx<T> *xw = reinterpret_cast<cx<T> *>(smem);                 // [rows][G] + 1 pad per box
+    T *kvs = reinterpret_cast<T *>(xw + (size_t)rows * G + BB + 3);  // [G][4b]
     const int sig = blockIdx.z;
     x += (int64_t)sig * x_stride;
     psi += (int64_t)sig * tree_stride + level_off(g, g.n);
@@ -454,7 +454,7 @@ __global__ __launch_bounds__(256) void k_leaf_mma(Geom g, const cx<T> *__restric
         mu = mu < 0 ? mu + M : (mu >= M ? mu - M : mu);
         cx<T> v = x[mu * P + sc0 + gg];
         if (conj) v.im = -v.im;
-        xw[e] = v;
+        xw[e + r / b] = v;  // one pad slot per box: box offsets land on distinct banks
     }
 #pragma unroll 8
     for (int e = threadIdx.x; e < G * 4 * b; e += blockDim.x) {
@@ -473,7 +473,7 @@ __global__ __launch_bounds__(256) void k_leaf_mma(Geom g, const cx<T> *__restric
         const int sc = sc0 + gg;
         if (sc == 0) {
             cx<T> *dst = psec + (int64_t)i0 * b;
-            for (int t = lane; t < BB * b; t += 64) dst[t] = xw[(2 * b + t) * G];
+            for (int t = lane; t < BB * b; t += 64) dst[t] = xw[(2 * b + t) * G + (2 * b + t) / b];
             continue;
         }
         const T *kr_ = kvs + gg * 4 * b;
@@ -495,17 +495,40 @@ __global__ __launch_bounds__(256) void k_leaf_mma(Geom g, const cx<T> *__restric
         V4 acc[NRT];
 #pragma unroll
         for (int rt = 0; rt < NRT; ++rt) acc[rt] = V4{0, 0, 0, 0};
-        // near field: B row q = 4 s + kr of box bx's window (rows (bx+1) b + q)
-        const T *xb = xwc + ((size_t)(bx + 1) * b * G + gg) * 2 + c;
-        constexpr int ksteps = 3 * b / 4;
-#pragma unroll 4
-        for (int s = 0; s < ksteps; ++s) {
-            const int q = 4 * s + kr;
-            const T bv = xb[(size_t)q * G * 2];
+        // near field.  K step s covers window row q = 4 s + kr of box bx
+        // (rows (bx+1) b + q); A(rt, s) = K index 4 s + kr + b - 16 rt - col.
+        // The Toeplitz structure gives A(rt+1, s+4) = A(rt, s), so each group
+        // of four steps loads only the rt = 0 values and shifts the rest.
+        constexpr int NG = 3 * b / 16;  // groups of four K steps
+        T areg[NRT][4];
 #pragma unroll
-            for (int rt = 0; rt < NRT; ++rt) {
-                const T av = kr_[q + b - 16 * rt - col];
-                acc[rt] = Mfma<T>::mma(av, bv, acc[rt]);
+        for (int rt = 0; rt < NRT; ++rt)
+#pragma unroll
+            for (int t = 0; t < 4; ++t) areg[rt][t] = kr_[4 * t + kr + b - 16 * rt - col];
+#pragma unroll
+        for (int gi = 0; gi < NG; ++gi) {
+            const int w = gi / (b / 16);        // window box 0..2
+            const int q0 = 16 * gi;             // first window row of the group
+            const T *xg = xwc + (((size_t)(bx + 1) * b + q0) * G + gg + (bx + 1 + w)) * 2 + c;
+            T bv[4];
+#pragma unroll
+            for (int t = 0; t < 4; ++t) bv[t] = xg[(size_t)(4 * t + kr) * G * 2];
+            T an[4];
+            if (gi + 1 < NG) {
+#pragma unroll
+                for (int t = 0; t < 4; ++t) an[t] = kr_[16 * (gi + 1) + 4 * t + kr + b - col];
+            }
+#pragma unroll
+            for (int t = 0; t < 4; ++t)
+#pragma unroll
+                for (int rt = 0; rt < NRT; ++rt) acc[rt] = Mfma<T>::mma(areg[rt][t], bv[t], acc[rt]);
+            if (gi + 1 < NG) {
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+#pragma unroll
+                    for (int rt = NRT - 1; rt > 0; --rt) areg[rt][t] = areg[rt - 1][t];
+                    areg[0][t] = an[t];
+                }
             }
         }
         // far field (L2T): A = evalm rows, B = local expansion of box bx
@@ -519,8 +542,8 @@ __global__ __launch_bounds__(256) void k_leaf_mma(Geom g, const cx<T> *__restric
             // re-home target 0 of box bx to box bx-1: lanes col, col+16, col+32,
             // col+48 each take a quarter of the correction terms
             T corr = 0;
-            const T *xm2 = xwc + ((size_t)bx * b * G + gg) * 2 + c;           // box i-2
-            const T *xp1 = xwc + ((size_t)(bx + 3) * b * G + gg) * 2 + c;     // box i+1
+            const T *xm2 = xwc + ((size_t)bx * b * G + gg + bx) * 2 + c;                // box i-2
+            const T *xp1 = xwc + ((size_t)(bx + 3) * b * G + gg + bx + 3) * 2 + c;      // box i+1
             for (int t = kr; t < b; t += 4)
                 corr += kr_[t] * xm2[(size_t)t * G * 2] - kr_[3 * b + t] * xp1[(size_t)t * G * 2];
             const T *locm = psi + (int64_t)(pmod(i0 + bx - 1, g.s)) * terms * rowsS + 2 * (sc - 1) + c;
