@@ -73,7 +73,7 @@ int validate(int64_t n, int32_t P, int32_t terms, int32_t leaf, int32_t dtype, P
 // Offsets (in doubles) of the FP64 operator tables; the FP32 copies use the
 // same offsets in floats for the first `n_ops` entries.
 struct TableLayout {
-    size_t cheb, initch, ops, topflp, evalm, evalmh, kv, n_ops;  // contiguous block
+    size_t cheb, initch, ops, topflp, evalm, evalmh, kv, evfrag, n_ops;  // contiguous block
     size_t gfac, wP;                                             // complex128 [P]
     size_t twA, twB, tlo, thi;                                   // complex128
     size_t total;                                                // doubles
@@ -112,6 +112,7 @@ TableLayout table_layout(const Geom &g, const FftLayout &f) {
     t.evalm = o; o += (size_t)g.S * g.b * T;
     t.evalmh = o; o += (size_t)(g.P / 2) * T;
     t.kv = o; o += (size_t)g.S * 4 * g.b;
+    t.evfrag = o; o += (g.b % 16 == 0) ? (size_t)g.S * (g.b / 16) * ((g.terms + 3) / 4) * 64 : 0;
     t.n_ops = o;
     o = (o + 1) & ~size_t(1);
     t.gfac = o; o += 2 * (size_t)g.P;
@@ -143,7 +144,8 @@ size_t leaf_lds(const Geom &g, size_t esize, int G, int bb) {
 }
 
 size_t leaf_mma_lds(const Geom &g, size_t esize, int G) {
-    return ((size_t)11 * g.b * G + 11) * 2 * esize + (size_t)G * 4 * g.b * esize;
+    return ((size_t)11 * g.b * G + 11) * 2 * esize + (size_t)G * 4 * g.b * esize +
+           (size_t)9 * (g.terms * 2 * G + 2) * esize;
 }
 
 Shapes shapes(const Geom &g, size_t esize) {
@@ -216,7 +218,7 @@ namespace {
 
 template <typename T>
 struct TabPtrs {
-    const T *initch, *ops, *topflp, *evalm, *evalmh, *kv, *twA, *twB;
+    const T *initch, *ops, *topflp, *evalm, *evalmh, *kv, *evfrag, *twA, *twB;
 };
 
 template <typename T>
@@ -225,14 +227,14 @@ TabPtrs<T> tab_ptrs(const fmmfft_plan_s *p);
 template <>
 TabPtrs<double> tab_ptrs<double>(const fmmfft_plan_s *p) {
     const double *t = p->tab64;
-    return {t + p->tl.initch, t + p->tl.ops,   t + p->tl.topflp, t + p->tl.evalm,
-            t + p->tl.evalmh, t + p->tl.kv,    t + p->tl.twA,    t + p->tl.twB};
+    return {t + p->tl.initch, t + p->tl.ops, t + p->tl.topflp, t + p->tl.evalm, t + p->tl.evalmh,
+            t + p->tl.kv,     t + p->tl.evfrag, t + p->tl.twA, t + p->tl.twB};
 }
 template <>
 TabPtrs<float> tab_ptrs<float>(const fmmfft_plan_s *p) {
     const float *t = p->tab32;
-    return {t + p->tl.initch, t + p->tl.ops,   t + p->tl.topflp, t + p->tl.evalm,
-            t + p->tl.evalmh, t + p->tl.kv,    t + p->tw32_A,    t + p->tw32_B};
+    return {t + p->tl.initch, t + p->tl.ops, t + p->tl.topflp, t + p->tl.evalm, t + p->tl.evalmh,
+            t + p->tl.kv,     t + p->tl.evfrag, t + p->tw32_A, t + p->tw32_B};
 }
 
 inline unsigned blocks_for(int64_t n, int per = 256) { return (unsigned)((n + per - 1) / per); }
@@ -350,7 +352,7 @@ int run(const fmmfft_plan_s *p, int64_t count, const void *d_in, void *d_out, in
         const unsigned nwg = (unsigned)((g.P / sh.leaf_G) * (g.s / 8));
 #define LEAF_MMA(NRT)                                                                                        \
     hipLaunchKernelGGL((k_leaf_mma<T, NRT>), dim3(nwg, 1, nz), dim3(256), sh.leaf_lds, st, g, x, g.N, psi, \
-                       wstride_T, psec, wstride_cx, tb.evalm, tb.evalmh, tb.kv, sh.leaf_G, conj)
+                       wstride_T, psec, wstride_cx, tb.evfrag, tb.evalm, tb.evalmh, tb.kv, sh.leaf_G, conj)
         switch (g.b / 16) {
             case 1: LEAF_MMA(1); break;
             case 2: LEAF_MMA(2); break;
@@ -475,6 +477,11 @@ int fmmfft_plan_create(fmmfft_plan *out, int64_t n, int32_t p_blocks, int32_t te
     const int jobs = 8 * (g.n - 2) + 1 + g.S * g.b + g.P / 2;
     hipLaunchKernelGGL(k_build_interp, dim3(jobs), dim3(64), 0, st, g, t + tl.ops, t + tl.topflp,
                        t + tl.evalm, t + tl.evalmh);
+    if (g.b % 16 == 0) {
+        const int nrt = g.b / 16, kt = (g.terms + 3) / 4;
+        hipLaunchKernelGGL(k_build_evfrag, dim3(blocks_for((int64_t)g.S * nrt * kt * 64)), dim3(256), 0, st, g,
+                           (const double *)(t + tl.evalm), t + tl.evfrag, nrt, kt);
+    }
     const int64_t R = int64_t(1) << p->fft.logR;
     const int64_t MA = p->fft.M1, MB = p->fft.M2;
     hipLaunchKernelGGL(k_build_twiddles, dim3(blocks_for(MA + R + g.M / R)), dim3(256), 0, st, MA, g.M, R,

@@ -421,18 +421,23 @@ template <typename T, int NRT>
 __global__ __launch_bounds__(256) void k_leaf_mma(Geom g, const cx<T> *__restrict__ x, int64_t x_stride,
                                                   const T *__restrict__ psi, int64_t tree_stride,
                                                   cx<T> *__restrict__ psec, int64_t psec_stride,
-                                                  const T *__restrict__ evalm, const T *__restrict__ evalmh,
-                                                  const T *__restrict__ kv, int G, int conj) {
+                                                  const T *__restrict__ evfrag, const T *__restrict__ evalm,
+                                                  const T *__restrict__ evalmh, const T *__restrict__ kv, int G,
+                                                  int conj) {
     using V4 = typename Mfma<T>::v4;
     constexpr int BB = 8;  // boxes per workgroup = MFMA columns / 2
-    extern __shared__ __align__(16) unsigned char smem[];
     constexpr int b = 16 * NRT;
+    constexpr int KT = 8;  // K steps of the L2T product (terms <= 32)
+    extern __shared__ __align__(16) unsigned char smem[];
     const int P = g.P, terms = g.terms, S = g.S;
     const int64_t M = g.M;
     const int rows = (BB + 3) * b;
-    const int rowsS = 2 * S;  // tree row stride
-    cx<T> *xw = reinterpret_cast<cx<T> *>(smem);                 // [rows][G] + 1 pad per box
+    const int rowsS = 2 * S;          // tree row stride
+    const int kt = (terms + 3) / 4;
+    const int lstr = terms * 2 * G + 2;  // LDS stride of one box's local expansions (+2: bank shift)
+    cx<T> *xw = reinterpret_cast<cx<T> *>(smem);                     // [rows][G] + 1 pad per box
     T *kvs = reinterpret_cast<T *>(xw + (size_t)rows * G + BB + 3);  // [G][4b]
+    T *lsx = kvs + (size_t)G * 4 * b;                                // [BB+1][terms][G][2]
     const int sig = blockIdx.z;
     x += (int64_t)sig * x_stride;
     psi += (int64_t)sig * tree_stride + level_off(g, g.n);
@@ -446,6 +451,8 @@ __global__ __launch_bounds__(256) void k_leaf_mma(Geom g, const cx<T> *__restric
     const int sc0 = (lid % ngroups) * G;
     const int i0 = (lid / ngroups) * BB;
 
+    // stage: x window (boxes i0-2 .. i0+8), kernel samples, local expansions
+    // of boxes i0-1 .. i0+7 (all coalesced)
     const int64_t mu_w0 = (int64_t)(i0 - 2) * b;
 #pragma unroll 8
     for (int e = threadIdx.x; e < rows * G; e += blockDim.x) {
@@ -462,6 +469,16 @@ __global__ __launch_bounds__(256) void k_leaf_mma(Geom g, const cx<T> *__restric
         const int sc = sc0 + gg;
         kvs[e] = sc == 0 ? T(0) : kv[(size_t)(sc - 1) * 4 * b + d];
     }
+    const int nloc = terms * 2 * G;
+#pragma unroll 4
+    for (int e = threadIdx.x; e < (BB + 1) * nloc; e += blockDim.x) {
+        const int bx9 = e / nloc, rem = e - bx9 * nloc;
+        const int k = rem / (2 * G), rr = rem - k * 2 * G;
+        const int sc = sc0 + (rr >> 1);
+        const int box = pmod(i0 - 1 + bx9, g.s);
+        lsx[bx9 * lstr + rem] =
+            sc == 0 ? T(0) : psi[((int64_t)box * terms + k) * rowsS + 2 * (sc - 1) + (rr & 1)];
+    }
     __syncthreads();
 
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -477,21 +494,13 @@ __global__ __launch_bounds__(256) void k_leaf_mma(Geom g, const cx<T> *__restric
             continue;
         }
         const T *kr_ = kvs + gg * 4 * b;
-        // issue the L2T operand loads (global) now; they land while the
-        // near-field MFMAs run
-        const T *loc = psi + (int64_t)(pmod(i0 + bx, g.s)) * terms * rowsS + 2 * (sc - 1) + c;
-        const T *ev = evalm + (size_t)(sc - 1) * b * terms;
-        constexpr int KT = 8;  // terms <= 32
-        const int kt = (terms + 3) / 4;
-        T la[NRT][KT], lb[KT];
+        // L2T A fragments, stored in lane order: one coalesced load per step
+        T la[NRT][KT];
 #pragma unroll
-        for (int s = 0; s < KT; ++s) {
-            const int k = 4 * s + kr;
-            const bool ok = s < kt && k < terms;
-            lb[s] = ok ? loc[(int64_t)k * rowsS] : T(0);
+        for (int s2 = 0; s2 < KT; ++s2)
 #pragma unroll
-            for (int rt = 0; rt < NRT; ++rt) la[rt][s] = ok ? ev[(16 * rt + col) * terms + k] : T(0);
-        }
+            for (int rt = 0; rt < NRT; ++rt)
+                la[rt][s2] = s2 < kt ? evfrag[((((size_t)(sc - 1) * NRT + rt) * kt + s2) << 6) + lane] : T(0);
         V4 acc[NRT];
 #pragma unroll
         for (int rt = 0; rt < NRT; ++rt) acc[rt] = V4{0, 0, 0, 0};
@@ -507,8 +516,8 @@ __global__ __launch_bounds__(256) void k_leaf_mma(Geom g, const cx<T> *__restric
             for (int t = 0; t < 4; ++t) areg[rt][t] = kr_[4 * t + kr + b - 16 * rt - col];
 #pragma unroll
         for (int gi = 0; gi < NG; ++gi) {
-            const int w = gi / (b / 16);        // window box 0..2
-            const int q0 = 16 * gi;             // first window row of the group
+            const int w = gi / (b / 16);  // window box 0..2
+            const int q0 = 16 * gi;       // first window row of the group
             const T *xg = xwc + (((size_t)(bx + 1) * b + q0) * G + gg + (bx + 1 + w)) * 2 + c;
             T bv[4];
 #pragma unroll
@@ -531,35 +540,52 @@ __global__ __launch_bounds__(256) void k_leaf_mma(Geom g, const cx<T> *__restric
                 }
             }
         }
-        // far field (L2T): A = evalm rows, B = local expansion of box bx
+        // far field (L2T): B = local expansion of box bx from LDS
+        const T *lb = lsx + (bx + 1) * lstr + 2 * gg + c;
 #pragma unroll
-        for (int s = 0; s < KT; ++s)
-            if (s < kt) {
+        for (int s2 = 0; s2 < KT; ++s2)
+            if (s2 < kt) {
+                const int k = 4 * s2 + kr;
+                const T bvl = k < terms ? lb[k * 2 * G] : T(0);
 #pragma unroll
-                for (int rt = 0; rt < NRT; ++rt) acc[rt] = Mfma<T>::mma(la[rt][s], lb[s], acc[rt]);
+                for (int rt = 0; rt < NRT; ++rt) acc[rt] = Mfma<T>::mma(la[rt][s2], bvl, acc[rt]);
             }
         if (2 * sc >= P) {
             // re-home target 0 of box bx to box bx-1: lanes col, col+16, col+32,
             // col+48 each take a quarter of the correction terms
             T corr = 0;
-            const T *xm2 = xwc + ((size_t)bx * b * G + gg + bx) * 2 + c;                // box i-2
-            const T *xp1 = xwc + ((size_t)(bx + 3) * b * G + gg + bx + 3) * 2 + c;      // box i+1
+            const T *xm2 = xwc + ((size_t)bx * b * G + gg + bx) * 2 + c;            // box i-2
+            const T *xp1 = xwc + ((size_t)(bx + 3) * b * G + gg + bx + 3) * 2 + c;  // box i+1
             for (int t = kr; t < b; t += 4)
                 corr += kr_[t] * xm2[(size_t)t * G * 2] - kr_[3 * b + t] * xp1[(size_t)t * G * 2];
-            const T *locm = psi + (int64_t)(pmod(i0 + bx - 1, g.s)) * terms * rowsS + 2 * (sc - 1) + c;
+            const T *lm = lsx + bx * lstr + 2 * gg + c;  // box i-1
             const T *eh = evalmh + (size_t)(sc - P / 2) * terms;
-            for (int k = kr; k < terms; k += 4)
-                corr += eh[k] * locm[(int64_t)k * rowsS] - ev[k] * loc[(int64_t)k * rowsS];
+            const T *e0 = evalm + (size_t)(sc - 1) * b * terms;  // row of target 0
+            for (int k = kr; k < terms; k += 4) corr += eh[k] * lm[k * 2 * G] - e0[k] * lb[k * 2 * G];
             corr += __shfl_xor(corr, 16, 64);
             corr += __shfl_xor(corr, 32, 64);
             if (kr == 0) acc[0][0] += (2 * sc == P) ? T(0.5) * corr : corr;
         }
-        // write: lane holds targets 16 rt + row(kr, r) of box bx, component c
-        T *dst = reinterpret_cast<T *>(psec + (int64_t)sc * M + (int64_t)(i0 + bx) * b) + c;
+        // write: pair lanes (c = 0, 1) swap halves so that each lane holds two
+        // complete complex outputs; 8 lanes then cover 16 consecutive targets
+        cx<T> *dst = psec + (int64_t)sc * M + (int64_t)(i0 + bx) * b;
 #pragma unroll
-        for (int rt = 0; rt < NRT; ++rt)
-#pragma unroll
-            for (int r = 0; r < 4; ++r) dst[2 * (16 * rt + Mfma<T>::row(kr, r))] = acc[rt][r];
+        for (int rt = 0; rt < NRT; ++rt) {
+            T send0 = c ? acc[rt][0] : acc[rt][2];
+            T send1 = c ? acc[rt][1] : acc[rt][3];
+            const T recv0 = __shfl_xor(send0, 1, 64);
+            const T recv1 = __shfl_xor(send1, 1, 64);
+            const cx<T> o0 = c ? cx<T>{recv0, acc[rt][2]} : cx<T>{acc[rt][0], recv0};
+            const cx<T> o1 = c ? cx<T>{recv1, acc[rt][3]} : cx<T>{acc[rt][1], recv1};
+            if constexpr (sizeof(T) == 4) {
+                // rows 4 kr + 2c, 4 kr + 2c + 1: one 16-byte store
+                *reinterpret_cast<float4 *>(dst + 16 * rt + Mfma<T>::row(kr, 2 * c)) =
+                    make_float4(o0.re, o0.im, o1.re, o1.im);
+            } else {
+                dst[16 * rt + Mfma<T>::row(kr, 2 * c)] = o0;
+                dst[16 * rt + Mfma<T>::row(kr, 2 * c + 1)] = o1;
+            }
+        }
     }
 }
 

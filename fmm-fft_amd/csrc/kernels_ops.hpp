@@ -212,6 +212,25 @@ __global__ __launch_bounds__(256) void k_build_twiddles(int64_t L, int64_t M, in
     }
 }
 
+// L2T rows in MFMA fragment order for the matrix-core leaf kernel:
+// evfrag[sc-1][rt][s][lane] = evalm[sc-1][16 rt + lane % 16][4 s + lane / 16]
+// (zero past the last term), so that one K step of one 16-row tile is one
+// coalesced 256-byte load.
+__global__ __launch_bounds__(256) void k_build_evfrag(Geom g, const double *evalm, double *evfrag, int nrt,
+                                                       int kt) {
+    const int64_t n = (int64_t)g.S * nrt * kt * 64;
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    const int lane = (int)(e & 63);
+    int64_t r = e >> 6;
+    const int s = (int)(r % kt);
+    r /= kt;
+    const int rt = (int)(r % nrt);
+    const int64_t sc1 = r / nrt;
+    const int k = 4 * s + lane / 16, j = 16 * rt + lane % 16;
+    evfrag[e] = k < g.terms ? evalm[(sc1 * g.b + j) * g.terms + k] : 0.0;
+}
+
 __global__ __launch_bounds__(256) void k_to_f32(const double *src, float *dst, int64_t n) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) dst[i] = (float)src[i];
