@@ -147,6 +147,7 @@ size_t leaf_mma_lds(const Geom &g, size_t esize, int G) {
     return ((size_t)11 * g.b * G + 11) * 2 * esize + (size_t)G * 4 * g.b * esize +
            (size_t)9 * (g.terms * 2 * G + 2) * esize;
 }
+// (the persistent leaf kernel's LDS is at most 11*128*4*16 + ... < 100 KB)
 
 Shapes shapes(const Geom &g, size_t esize) {
     Shapes sh{};
@@ -166,8 +167,8 @@ Shapes shapes(const Geom &g, size_t esize) {
     sh.leaf_mma = g.b % 16 == 0 && g.b <= 128;  // b is a power of two: b/16 in {1,2,4,8}
     if (sh.leaf_mma) {
         // one 64-byte x segment per window row, 8 boxes per workgroup
-        int Gm = std::min(g.P, esize == 4 ? 8 : 4);
-        while (leaf_mma_lds(g, esize, Gm) > lds_cap && Gm > 1) Gm >>= 1;
+        // 4 sections per tile (2 when P = 2): one section per wave
+        int Gm = std::min(g.P, 4);
         sh.leaf_G = Gm;
         sh.leaf_bb = 8;
         sh.leaf_lds = leaf_mma_lds(g, esize, Gm);
@@ -279,6 +280,53 @@ void launch_combine(const Geom &g, unsigned nz, hipStream_t st, const cx<T> *pse
 #undef COMBINE
 }
 
+template <typename T, int NRT, int G, int KT>
+void leaf_mma_inst(const Geom &g, const Shapes &sh, unsigned nz, hipStream_t st, const cx<T> *x, const T *psi,
+                   int64_t wstride_T, cx<T> *psec, int64_t wstride_cx, const TabPtrs<T> &tb, int conj) {
+    auto kern = k_leaf_mma<T, NRT, G, KT>;
+    // persistent grid: as many workgroups as can be resident at once
+    int dev = 0, cus = 0, per_cu = 0;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (sh.leaf_lds > 64 * 1024)
+        (void)hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh.leaf_lds);
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 256, sh.leaf_lds) != hipSuccess || per_cu < 1)
+        per_cu = 1;
+    const int64_t ntiles = (int64_t)(g.P / G) * (g.s / 8);
+    const unsigned nwg = (unsigned)std::min<int64_t>(ntiles, (int64_t)std::max(cus, 8) * per_cu);
+    hipLaunchKernelGGL(kern, dim3(nwg, 1, nz), dim3(256), sh.leaf_lds, st, g, x, g.N, psi, wstride_T, psec,
+                       wstride_cx, tb.evfrag, tb.evalm, tb.evalmh, tb.kv, conj);
+}
+
+template <typename T, int NRT, int G>
+void leaf_mma_kt(const Geom &g, const Shapes &sh, unsigned nz, hipStream_t st, const cx<T> *x, const T *psi,
+                 int64_t wstride_T, cx<T> *psec, int64_t wstride_cx, const TabPtrs<T> &tb, int conj) {
+    if (g.terms <= 16)
+        leaf_mma_inst<T, NRT, G, 4>(g, sh, nz, st, x, psi, wstride_T, psec, wstride_cx, tb, conj);
+    else
+        leaf_mma_inst<T, NRT, G, 8>(g, sh, nz, st, x, psi, wstride_T, psec, wstride_cx, tb, conj);
+}
+
+template <typename T, int NRT>
+void leaf_mma_g(const Geom &g, const Shapes &sh, unsigned nz, hipStream_t st, const cx<T> *x, const T *psi,
+                int64_t wstride_T, cx<T> *psec, int64_t wstride_cx, const TabPtrs<T> &tb, int conj) {
+    if (sh.leaf_G >= 4)
+        leaf_mma_kt<T, NRT, 4>(g, sh, nz, st, x, psi, wstride_T, psec, wstride_cx, tb, conj);
+    else
+        leaf_mma_kt<T, NRT, 2>(g, sh, nz, st, x, psi, wstride_T, psec, wstride_cx, tb, conj);
+}
+
+template <typename T>
+void launch_leaf_mma(const Geom &g, const Shapes &sh, unsigned nz, hipStream_t st, const cx<T> *x, const T *psi,
+                     int64_t wstride_T, cx<T> *psec, int64_t wstride_cx, const TabPtrs<T> &tb, int conj) {
+    switch (g.b / 16) {
+        case 1: leaf_mma_g<T, 1>(g, sh, nz, st, x, psi, wstride_T, psec, wstride_cx, tb, conj); break;
+        case 2: leaf_mma_g<T, 2>(g, sh, nz, st, x, psi, wstride_T, psec, wstride_cx, tb, conj); break;
+        case 4: leaf_mma_g<T, 4>(g, sh, nz, st, x, psi, wstride_T, psec, wstride_cx, tb, conj); break;
+        default: leaf_mma_g<T, 8>(g, sh, nz, st, x, psi, wstride_T, psec, wstride_cx, tb, conj); break;
+    }
+}
+
 template <typename T, int PT, typename Mark>
 void s2m_tree_pt(const Geom &g, const Shapes &sh, unsigned nz, hipStream_t st, const cx<T> *x, T *phi, T *psi,
                  int64_t wstride_T, c64 *sums_part, int64_t wstride_c64, c64 *sumsec, const TabPtrs<T> &tb,
@@ -349,17 +397,7 @@ int run(const fmmfft_plan_s *p, int64_t count, const void *d_in, void *d_out, in
                        sumsec, tb, conj, mark);
     mark(3);
     if (sh.leaf_mma) {
-        const unsigned nwg = (unsigned)((g.P / sh.leaf_G) * (g.s / 8));
-#define LEAF_MMA(NRT)                                                                                        \
-    hipLaunchKernelGGL((k_leaf_mma<T, NRT>), dim3(nwg, 1, nz), dim3(256), sh.leaf_lds, st, g, x, g.N, psi, \
-                       wstride_T, psec, wstride_cx, tb.evfrag, tb.evalm, tb.evalmh, tb.kv, sh.leaf_G, conj)
-        switch (g.b / 16) {
-            case 1: LEAF_MMA(1); break;
-            case 2: LEAF_MMA(2); break;
-            case 4: LEAF_MMA(4); break;
-            default: LEAF_MMA(8); break;
-        }
-#undef LEAF_MMA
+        launch_leaf_mma<T>(g, sh, nz, st, x, psi, wstride_T, psec, wstride_cx, tb, conj);
     } else {
         hipLaunchKernelGGL(k_leaf<T>, dim3(g.s / sh.leaf_bb, g.P / sh.leaf_G, nz), dim3(256), sh.leaf_lds, st, g,
                            x, g.N, psi, wstride_T, psec, wstride_cx, tb.evalm, tb.evalmh, tb.kv, sh.leaf_G,

@@ -388,10 +388,12 @@ __global__ __launch_bounds__(256) void k_leaf(Geom g, const cx<T> *__restrict__ 
 // i-1 by adding the difference of the two evaluations (two b-long kernel dot
 // products and two L2T rows), in full for sc > P/2 and halved for sc = P/2.
 //
-// Workgroup: 4 waves, 8 boxes x G sections (one section per wave at a time).
-// LDS: the x window of boxes i0-2 .. i0+8 for the G sections and their
-// kernel samples.  The workgroup index is remapped so that workgroups that
-// share x lines (all section groups of one box chunk) run on one XCD.
+// Tile: 8 boxes x G sections (one section per wave at a time).  The kernel
+// is persistent: a workgroup loops over tiles and issues all global loads of
+// its next tile (x window of boxes i0-2 .. i0+8, kernel samples, local
+// expansions of boxes i0-1 .. i0+7) into registers before computing the
+// current one, so HBM latency hides behind the MFMAs.  Tiles are ordered so
+// that the workgroups of one XCD share x lines.
 // MFMA lane layout (16x16x4): lane l supplies A[l%16][l/16] and B[l/16][l%16]
 // and receives D[row][l%16] for its four accumulators r = 0..3, with
 // row = 4*(l/16) + r for FP32 and l/16 + 4*r for FP64.
@@ -417,175 +419,223 @@ struct Mfma<double> {
     static __device__ inline int row(int kr, int r) { return kr + 4 * r; }
 };
 
-template <typename T, int NRT>
+template <typename T, int NRT, int G, int KT>
 __global__ __launch_bounds__(256) void k_leaf_mma(Geom g, const cx<T> *__restrict__ x, int64_t x_stride,
                                                   const T *__restrict__ psi, int64_t tree_stride,
                                                   cx<T> *__restrict__ psec, int64_t psec_stride,
                                                   const T *__restrict__ evfrag, const T *__restrict__ evalm,
-                                                  const T *__restrict__ evalmh, const T *__restrict__ kv, int G,
-                                                  int conj) {
+                                                  const T *__restrict__ evalmh, const T *__restrict__ kv, int conj) {
     using V4 = typename Mfma<T>::v4;
-    constexpr int BB = 8;  // boxes per workgroup = MFMA columns / 2
-    constexpr int b = 16 * NRT;
-    constexpr int KT = 8;  // K steps of the L2T product (terms <= 32)
+    constexpr int BB = 8;            // boxes per tile = MFMA columns / 2
+    constexpr int b = 16 * NRT;      // leaf size
+    constexpr int ROWS = (BB + 3) * b;
+    constexpr int XN = ROWS * G, XPER = (XN + 255) / 256;           // x window
+    constexpr int KVN = G * 4 * b, KPER = (KVN + 255) / 256;        // kernel samples
+    constexpr int LMAX = (BB + 1) * 4 * KT * 2 * G, LPER = (LMAX + 255) / 256;  // local expansions
     extern __shared__ __align__(16) unsigned char smem[];
     const int P = g.P, terms = g.terms, S = g.S;
     const int64_t M = g.M;
-    const int rows = (BB + 3) * b;
-    const int rowsS = 2 * S;          // tree row stride
-    const int kt = (terms + 3) / 4;
-    const int lstr = terms * 2 * G + 2;  // LDS stride of one box's local expansions (+2: bank shift)
-    cx<T> *xw = reinterpret_cast<cx<T> *>(smem);                     // [rows][G] + 1 pad per box
-    T *kvs = reinterpret_cast<T *>(xw + (size_t)rows * G + BB + 3);  // [G][4b]
-    T *lsx = kvs + (size_t)G * 4 * b;                                // [BB+1][terms][G][2]
+    const int rowsS = 2 * S;             // tree row stride
+    const int kt = (terms + 3) / 4;      // L2T K steps (<= KT)
+    const int nloc = terms * 2 * G;
+    const int lstr = nloc + 2;           // LDS stride of one box's local expansions (+2: bank shift)
+    cx<T> *xw = reinterpret_cast<cx<T> *>(smem);           // [ROWS][G] + 1 pad per box
+    T *kvs = reinterpret_cast<T *>(xw + XN + BB + 3);      // [G][4b]
+    T *lsx = kvs + KVN;                                    // [BB+1][terms][G][2] (+2 per box)
     const int sig = blockIdx.z;
     x += (int64_t)sig * x_stride;
     psi += (int64_t)sig * tree_stride + level_off(g, g.n);
     psec += (int64_t)sig * psec_stride;
 
-    // XCD-aware order: consecutive logical indices share an XCD
+    // Persistent tiles.  Iteration k of workgroup w takes tile k*nwg + xo(w);
+    // xo makes the workgroups of one XCD take consecutive tiles (the section
+    // groups of one box chunk share x lines).
     const int ngroups = P / G;
-    const int nwg = gridDim.x * gridDim.y;
-    const int bid = blockIdx.y * gridDim.x + blockIdx.x;
-    const int lid = (nwg % 8 == 0) ? (bid % 8) * (nwg / 8) + bid / 8 : bid;
-    const int sc0 = (lid % ngroups) * G;
-    const int i0 = (lid / ngroups) * BB;
+    const int ntiles = ngroups * (g.s / BB);
+    const int nwg = gridDim.x, w = blockIdx.x;
+    const int xo = (nwg % 8 == 0) ? (w % 8) * (nwg / 8) + w / 8 : w;
 
-    // stage: x window (boxes i0-2 .. i0+8), kernel samples, local expansions
-    // of boxes i0-1 .. i0+7 (all coalesced)
-    const int64_t mu_w0 = (int64_t)(i0 - 2) * b;
-#pragma unroll 8
-    for (int e = threadIdx.x; e < rows * G; e += blockDim.x) {
-        const int r = e / G, gg = e - r * G;
-        int64_t mu = mu_w0 + r;
-        mu = mu < 0 ? mu + M : (mu >= M ? mu - M : mu);
-        cx<T> v = x[mu * P + sc0 + gg];
-        if (conj) v.im = -v.im;
-        xw[e + r / b] = v;  // one pad slot per box: box offsets land on distinct banks
-    }
-#pragma unroll 8
-    for (int e = threadIdx.x; e < G * 4 * b; e += blockDim.x) {
-        const int gg = e / (4 * b), d = e - gg * 4 * b;
-        const int sc = sc0 + gg;
-        kvs[e] = sc == 0 ? T(0) : kv[(size_t)(sc - 1) * 4 * b + d];
-    }
-    const int nloc = terms * 2 * G;
-#pragma unroll 4
-    for (int e = threadIdx.x; e < (BB + 1) * nloc; e += blockDim.x) {
-        const int bx9 = e / nloc, rem = e - bx9 * nloc;
-        const int k = rem / (2 * G), rr = rem - k * 2 * G;
-        const int sc = sc0 + (rr >> 1);
-        const int box = pmod(i0 - 1 + bx9, g.s);
-        lsx[bx9 * lstr + rem] =
-            sc == 0 ? T(0) : psi[((int64_t)box * terms + k) * rowsS + 2 * (sc - 1) + (rr & 1)];
-    }
-    __syncthreads();
+    cx<T> xr[XPER];
+    T kvr[KPER], lr[LPER];
+    // issue every global load of a tile into registers (they stay in flight
+    // while the previous tile computes)
+    auto prefetch = [&](int lid) {
+        const int sc0 = (lid % ngroups) * G, i0 = (lid / ngroups) * BB;
+        const int64_t mu_w0 = (int64_t)(i0 - 2) * b;
+#pragma unroll
+        for (int u = 0; u < XPER; ++u) {
+            const int e = threadIdx.x + 256 * u;
+            if (XN % 256 == 0 || e < XN) {
+                const int r = e / G, gg = e % G;
+                int64_t mu = mu_w0 + r;
+                mu = mu < 0 ? mu + M : (mu >= M ? mu - M : mu);
+                xr[u] = x[mu * P + sc0 + gg];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < KPER; ++u) {
+            const int e = threadIdx.x + 256 * u;
+            if (KVN % 256 == 0 || e < KVN) {
+                const int gg = e / (4 * b), d = e % (4 * b);
+                const int sc = sc0 + gg;
+                kvr[u] = sc == 0 ? T(0) : kv[(size_t)(sc - 1) * 4 * b + d];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < LPER; ++u) {
+            const int e = threadIdx.x + 256 * u;
+            if (e < (BB + 1) * nloc) {
+                const int bx9 = e / nloc, rem = e - bx9 * nloc;
+                const int k = rem / (2 * G), rr = rem - k * 2 * G;
+                const int sc = sc0 + (rr >> 1);
+                const int box = pmod(i0 - 1 + bx9, g.s);
+                lr[u] = sc == 0 ? T(0) : psi[((int64_t)box * terms + k) * rowsS + 2 * (sc - 1) + (rr & 1)];
+            }
+        }
+    };
+    auto commit = [&]() {
+#pragma unroll
+        for (int u = 0; u < XPER; ++u) {
+            const int e = threadIdx.x + 256 * u;
+            if (XN % 256 == 0 || e < XN) {
+                cx<T> v = xr[u];
+                if (conj) v.im = -v.im;
+                xw[e + (e / G) / b] = v;  // one pad slot per box: box offsets land on distinct banks
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < KPER; ++u) {
+            const int e = threadIdx.x + 256 * u;
+            if (KVN % 256 == 0 || e < KVN) kvs[e] = kvr[u];
+        }
+#pragma unroll
+        for (int u = 0; u < LPER; ++u) {
+            const int e = threadIdx.x + 256 * u;
+            if (e < (BB + 1) * nloc) {
+                const int bx9 = e / nloc;
+                lsx[bx9 * lstr + (e - bx9 * nloc)] = lr[u];
+            }
+        }
+    };
 
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int col = lane & 15, kr = lane >> 4;
     const int bx = col >> 1, c = col & 1;
     const T *xwc = reinterpret_cast<const T *>(xw);
 
-    for (int gg = wave; gg < G; gg += 4) {
-        const int sc = sc0 + gg;
-        if (sc == 0) {
-            cx<T> *dst = psec + (int64_t)i0 * b;
-            for (int t = lane; t < BB * b; t += 64) dst[t] = xw[(2 * b + t) * G + (2 * b + t) / b];
-            continue;
-        }
-        const T *kr_ = kvs + gg * 4 * b;
-        // L2T A fragments, stored in lane order: one coalesced load per step
-        T la[NRT][KT];
+    int k = 0;
+    int lid = xo;
+    if (lid < ntiles) prefetch(lid);
+    while (lid < ntiles) {
+        __syncthreads();  // the previous tile is no longer read
+        commit();
+        __syncthreads();
+        const int nlid = (++k) * nwg + xo;
+        if (nlid < ntiles) prefetch(nlid);
+        const int sc0 = (lid % ngroups) * G, i0 = (lid / ngroups) * BB;
+
+        for (int gg = wave; gg < G; gg += 4) {
+            const int sc = sc0 + gg;
+            if (sc == 0) {
+                cx<T> *dst = psec + (int64_t)i0 * b;
+                for (int t = lane; t < BB * b; t += 64) dst[t] = xw[(2 * b + t) * G + (2 * b + t) / b];
+                continue;
+            }
+            const T *kr_ = kvs + gg * 4 * b;
+            // L2T A fragments, stored in lane order: one coalesced load per step
+            T la[NRT][KT];
 #pragma unroll
-        for (int s2 = 0; s2 < KT; ++s2)
+            for (int s2 = 0; s2 < KT; ++s2)
+#pragma unroll
+                for (int rt = 0; rt < NRT; ++rt)
+                    la[rt][s2] = s2 < kt ? evfrag[((((size_t)(sc - 1) * NRT + rt) * kt + s2) << 6) + lane] : T(0);
+            V4 acc[NRT];
+#pragma unroll
+            for (int rt = 0; rt < NRT; ++rt) acc[rt] = V4{0, 0, 0, 0};
+            // near field.  K step s covers window row q = 4 s + kr of box bx
+            // (rows (bx+1) b + q); A(rt, s) = K index 4 s + kr + b - 16 rt - col.
+            // The Toeplitz structure gives A(rt+1, s+4) = A(rt, s), so each
+            // group of four steps loads only the rt = 0 values and shifts the rest.
+            constexpr int NG = 3 * b / 16;  // groups of four K steps
+            T areg[NRT][4];
 #pragma unroll
             for (int rt = 0; rt < NRT; ++rt)
-                la[rt][s2] = s2 < kt ? evfrag[((((size_t)(sc - 1) * NRT + rt) * kt + s2) << 6) + lane] : T(0);
-        V4 acc[NRT];
 #pragma unroll
-        for (int rt = 0; rt < NRT; ++rt) acc[rt] = V4{0, 0, 0, 0};
-        // near field.  K step s covers window row q = 4 s + kr of box bx
-        // (rows (bx+1) b + q); A(rt, s) = K index 4 s + kr + b - 16 rt - col.
-        // The Toeplitz structure gives A(rt+1, s+4) = A(rt, s), so each group
-        // of four steps loads only the rt = 0 values and shifts the rest.
-        constexpr int NG = 3 * b / 16;  // groups of four K steps
-        T areg[NRT][4];
+                for (int t = 0; t < 4; ++t) areg[rt][t] = kr_[4 * t + kr + b - 16 * rt - col];
 #pragma unroll
-        for (int rt = 0; rt < NRT; ++rt)
+            for (int gi = 0; gi < NG; ++gi) {
+                const int wb = gi / (b / 16);  // window box 0..2
+                const int q0 = 16 * gi;        // first window row of the group
+                const T *xg = xwc + (((size_t)(bx + 1) * b + q0) * G + gg + (bx + 1 + wb)) * 2 + c;
+                T bv[4];
 #pragma unroll
-            for (int t = 0; t < 4; ++t) areg[rt][t] = kr_[4 * t + kr + b - 16 * rt - col];
+                for (int t = 0; t < 4; ++t) bv[t] = xg[(size_t)(4 * t + kr) * G * 2];
+                T an[4];
+                if (gi + 1 < NG) {
 #pragma unroll
-        for (int gi = 0; gi < NG; ++gi) {
-            const int w = gi / (b / 16);  // window box 0..2
-            const int q0 = 16 * gi;       // first window row of the group
-            const T *xg = xwc + (((size_t)(bx + 1) * b + q0) * G + gg + (bx + 1 + w)) * 2 + c;
-            T bv[4];
+                    for (int t = 0; t < 4; ++t) an[t] = kr_[16 * (gi + 1) + 4 * t + kr + b - col];
+                }
 #pragma unroll
-            for (int t = 0; t < 4; ++t) bv[t] = xg[(size_t)(4 * t + kr) * G * 2];
-            T an[4];
-            if (gi + 1 < NG) {
+                for (int t = 0; t < 4; ++t)
 #pragma unroll
-                for (int t = 0; t < 4; ++t) an[t] = kr_[16 * (gi + 1) + 4 * t + kr + b - col];
+                    for (int rt = 0; rt < NRT; ++rt) acc[rt] = Mfma<T>::mma(areg[rt][t], bv[t], acc[rt]);
+                if (gi + 1 < NG) {
+#pragma unroll
+                    for (int t = 0; t < 4; ++t) {
+#pragma unroll
+                        for (int rt = NRT - 1; rt > 0; --rt) areg[rt][t] = areg[rt - 1][t];
+                        areg[0][t] = an[t];
+                    }
+                }
             }
+            // far field (L2T): B = local expansion of box bx from LDS
+            const T *lb = lsx + (bx + 1) * lstr + 2 * gg + c;
 #pragma unroll
-            for (int t = 0; t < 4; ++t)
+            for (int s2 = 0; s2 < KT; ++s2)
+                if (s2 < kt) {
+                    const int kk = 4 * s2 + kr;
+                    const T bvl = kk < terms ? lb[kk * 2 * G] : T(0);
 #pragma unroll
-                for (int rt = 0; rt < NRT; ++rt) acc[rt] = Mfma<T>::mma(areg[rt][t], bv[t], acc[rt]);
-            if (gi + 1 < NG) {
+                    for (int rt = 0; rt < NRT; ++rt) acc[rt] = Mfma<T>::mma(la[rt][s2], bvl, acc[rt]);
+                }
+            if (2 * sc >= P) {
+                // re-home target 0 of box bx to box bx-1: lanes col, col+16,
+                // col+32, col+48 each take a quarter of the correction terms
+                T corr = 0;
+                const T *xm2 = xwc + ((size_t)bx * b * G + gg + bx) * 2 + c;            // box i-2
+                const T *xp1 = xwc + ((size_t)(bx + 3) * b * G + gg + bx + 3) * 2 + c;  // box i+1
+                for (int t = kr; t < b; t += 4)
+                    corr += kr_[t] * xm2[(size_t)t * G * 2] - kr_[3 * b + t] * xp1[(size_t)t * G * 2];
+                const T *lm = lsx + bx * lstr + 2 * gg + c;  // box i-1
+                const T *eh = evalmh + (size_t)(sc - P / 2) * terms;
+                const T *e0 = evalm + (size_t)(sc - 1) * b * terms;  // row of target 0
+                for (int kk = kr; kk < terms; kk += 4) corr += eh[kk] * lm[kk * 2 * G] - e0[kk] * lb[kk * 2 * G];
+                corr += __shfl_xor(corr, 16, 64);
+                corr += __shfl_xor(corr, 32, 64);
+                if (kr == 0) acc[0][0] += (2 * sc == P) ? T(0.5) * corr : corr;
+            }
+            // write: pair lanes (c = 0, 1) swap halves so that each lane holds
+            // two complete complex outputs; 8 lanes cover 16 consecutive targets
+            cx<T> *dst = psec + (int64_t)sc * M + (int64_t)(i0 + bx) * b;
 #pragma unroll
-                for (int t = 0; t < 4; ++t) {
-#pragma unroll
-                    for (int rt = NRT - 1; rt > 0; --rt) areg[rt][t] = areg[rt - 1][t];
-                    areg[0][t] = an[t];
+            for (int rt = 0; rt < NRT; ++rt) {
+                T send0 = c ? acc[rt][0] : acc[rt][2];
+                T send1 = c ? acc[rt][1] : acc[rt][3];
+                const T recv0 = __shfl_xor(send0, 1, 64);
+                const T recv1 = __shfl_xor(send1, 1, 64);
+                const cx<T> o0 = c ? cx<T>{recv0, acc[rt][2]} : cx<T>{acc[rt][0], recv0};
+                const cx<T> o1 = c ? cx<T>{recv1, acc[rt][3]} : cx<T>{acc[rt][1], recv1};
+                if constexpr (sizeof(T) == 4) {
+                    // rows 4 kr + 2c, 4 kr + 2c + 1: one 16-byte store
+                    *reinterpret_cast<float4 *>(dst + 16 * rt + Mfma<T>::row(kr, 2 * c)) =
+                        make_float4(o0.re, o0.im, o1.re, o1.im);
+                } else {
+                    dst[16 * rt + Mfma<T>::row(kr, 2 * c)] = o0;
+                    dst[16 * rt + Mfma<T>::row(kr, 2 * c + 1)] = o1;
                 }
             }
         }
-        // far field (L2T): B = local expansion of box bx from LDS
-        const T *lb = lsx + (bx + 1) * lstr + 2 * gg + c;
-#pragma unroll
-        for (int s2 = 0; s2 < KT; ++s2)
-            if (s2 < kt) {
-                const int k = 4 * s2 + kr;
-                const T bvl = k < terms ? lb[k * 2 * G] : T(0);
-#pragma unroll
-                for (int rt = 0; rt < NRT; ++rt) acc[rt] = Mfma<T>::mma(la[rt][s2], bvl, acc[rt]);
-            }
-        if (2 * sc >= P) {
-            // re-home target 0 of box bx to box bx-1: lanes col, col+16, col+32,
-            // col+48 each take a quarter of the correction terms
-            T corr = 0;
-            const T *xm2 = xwc + ((size_t)bx * b * G + gg + bx) * 2 + c;            // box i-2
-            const T *xp1 = xwc + ((size_t)(bx + 3) * b * G + gg + bx + 3) * 2 + c;  // box i+1
-            for (int t = kr; t < b; t += 4)
-                corr += kr_[t] * xm2[(size_t)t * G * 2] - kr_[3 * b + t] * xp1[(size_t)t * G * 2];
-            const T *lm = lsx + bx * lstr + 2 * gg + c;  // box i-1
-            const T *eh = evalmh + (size_t)(sc - P / 2) * terms;
-            const T *e0 = evalm + (size_t)(sc - 1) * b * terms;  // row of target 0
-            for (int k = kr; k < terms; k += 4) corr += eh[k] * lm[k * 2 * G] - e0[k] * lb[k * 2 * G];
-            corr += __shfl_xor(corr, 16, 64);
-            corr += __shfl_xor(corr, 32, 64);
-            if (kr == 0) acc[0][0] += (2 * sc == P) ? T(0.5) * corr : corr;
-        }
-        // write: pair lanes (c = 0, 1) swap halves so that each lane holds two
-        // complete complex outputs; 8 lanes then cover 16 consecutive targets
-        cx<T> *dst = psec + (int64_t)sc * M + (int64_t)(i0 + bx) * b;
-#pragma unroll
-        for (int rt = 0; rt < NRT; ++rt) {
-            T send0 = c ? acc[rt][0] : acc[rt][2];
-            T send1 = c ? acc[rt][1] : acc[rt][3];
-            const T recv0 = __shfl_xor(send0, 1, 64);
-            const T recv1 = __shfl_xor(send1, 1, 64);
-            const cx<T> o0 = c ? cx<T>{recv0, acc[rt][2]} : cx<T>{acc[rt][0], recv0};
-            const cx<T> o1 = c ? cx<T>{recv1, acc[rt][3]} : cx<T>{acc[rt][1], recv1};
-            if constexpr (sizeof(T) == 4) {
-                // rows 4 kr + 2c, 4 kr + 2c + 1: one 16-byte store
-                *reinterpret_cast<float4 *>(dst + 16 * rt + Mfma<T>::row(kr, 2 * c)) =
-                    make_float4(o0.re, o0.im, o1.re, o1.im);
-            } else {
-                dst[16 * rt + Mfma<T>::row(kr, 2 * c)] = o0;
-                dst[16 * rt + Mfma<T>::row(kr, 2 * c + 1)] = o1;
-            }
-        }
+        lid = nlid;
     }
 }
 
