@@ -562,29 +562,41 @@ __global__ __launch_bounds__(256) void k_leaf_mma(Geom g, const cx<T> *__restric
             for (int rt = 0; rt < NRT; ++rt)
 #pragma unroll
                 for (int t = 0; t < 4; ++t) areg[rt][t] = kr_[4 * t + kr + b - 16 * rt - col];
+            // window rows of group gi: (bx+1) b + 16 gi + 4 t + kr (+ box pad)
+            auto xrow = [&](int gi) {
+                return xwc + (((size_t)(bx + 1) * b + 16 * gi) * G + gg + (bx + 1 + gi / (b / 16))) * 2 + c;
+            };
+            // B loads run one group ahead of the MFMAs that use them
+            T bvc[4];
+            {
+                const T *xg = xrow(0);
+#pragma unroll
+                for (int t = 0; t < 4; ++t) bvc[t] = xg[(size_t)(4 * t + kr) * G * 2];
+            }
 #pragma unroll
             for (int gi = 0; gi < NG; ++gi) {
-                const int wb = gi / (b / 16);  // window box 0..2
-                const int q0 = 16 * gi;        // first window row of the group
-                const T *xg = xwc + (((size_t)(bx + 1) * b + q0) * G + gg + (bx + 1 + wb)) * 2 + c;
-                T bv[4];
-#pragma unroll
-                for (int t = 0; t < 4; ++t) bv[t] = xg[(size_t)(4 * t + kr) * G * 2];
-                T an[4];
+                T bvn[4], an[4];
                 if (gi + 1 < NG) {
+                    const T *xg = xrow(gi + 1);
 #pragma unroll
-                    for (int t = 0; t < 4; ++t) an[t] = kr_[16 * (gi + 1) + 4 * t + kr + b - col];
+                    for (int t = 0; t < 4; ++t) {
+                        bvn[t] = xg[(size_t)(4 * t + kr) * G * 2];
+                        an[t] = kr_[16 * (gi + 1) + 4 * t + kr + b - col];
+                    }
                 }
+                // keep the next group's loads ahead of this group's MFMAs
+                __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
                 for (int t = 0; t < 4; ++t)
 #pragma unroll
-                    for (int rt = 0; rt < NRT; ++rt) acc[rt] = Mfma<T>::mma(areg[rt][t], bv[t], acc[rt]);
+                    for (int rt = 0; rt < NRT; ++rt) acc[rt] = Mfma<T>::mma(areg[rt][t], bvc[t], acc[rt]);
                 if (gi + 1 < NG) {
 #pragma unroll
                     for (int t = 0; t < 4; ++t) {
 #pragma unroll
                         for (int rt = NRT - 1; rt > 0; --rt) areg[rt][t] = areg[rt - 1][t];
                         areg[0][t] = an[t];
+                        bvc[t] = bvn[t];
                     }
                 }
             }
