@@ -11,8 +11,10 @@ once, as the reference's mfti/mft timing loop does, src/fmmfft.c:1249-1275).
 Reported: GFLOP/s = 5 N log2 N / t of the complex64 path, the complex128
 ("plain FP64") path at the same N for the ratio, the rel-L2 distance between
 the two outputs, p50/p99 step latency, a per-stage breakdown from HIP events
-recorded on the execute stream, the roofline of the dominant kernel, and the
-CPU baseline (the oracle's FMM-FFT restatement timed on the host cores on a
+recorded on the execute stream, the roofline of the dominant kernel, a mixed
+request stream through the native server (saturation throughput and p50/p99
+latency at Poisson arrival rates of 50% and 90% of saturation), and the CPU
+baseline (the oracle's FMM-FFT restatement timed on the host cores on a
 bounded sample of the same workload).
 """
 import argparse
@@ -82,6 +84,97 @@ def pct(v, q):
     return v[min(len(v) - 1, int(math.ceil(q / 100 * len(v))) - 1)]
 
 
+def serving_shape(k, terms, c128):
+    """Serving-form request shape: N = 2^k, P = 2^min(6, k // 4), leaf 64
+    (32 when M < 1024 so that the tree keeps >= 3 levels)."""
+    n = 1 << k
+    p = 1 << min(6, k // 4)
+    leaf = 64 if n // p >= 1024 else 32
+    return (n, p, terms, leaf, 1 if c128 else 0)
+
+
+def serving_stream(torch, fmm, n_req, rate, seed, budget_bytes, pools, max_inflight=256):
+    """Mixed request stream through the native server (fmmfft_server_*):
+    k ~ U{12..24}, p in {12, 20}, 70% complex64 / 30% complex128.  rate =
+    None submits as fast as the in-flight cap allows (saturation); otherwise
+    arrivals are Poisson with that many requests per second.  Inputs come
+    from per-shape pools of seeded device buffers (two per shape); each
+    request writes its own output buffer."""
+    import random
+    import threading
+    import time as _t
+
+    rng = random.Random(seed)
+    reqs = []
+    for i in range(n_req):
+        k = rng.randint(12, 24)
+        shape = serving_shape(k, rng.choice((12, 20)), rng.random() < 0.3)
+        reqs.append((shape, rng.getrandbits(1)))
+    for shape, _ in reqs:
+        if shape not in pools:
+            dt = torch.complex128 if shape[4] == 1 else torch.complex64
+            g = torch.Generator(device="cuda").manual_seed(hash(shape) & 0xFFFFFFFF)
+            pools[shape] = [torch.randn(shape[0], dtype=dt, device="cuda", generator=g) for _ in range(2)]
+    srv = fmm.Server(budget_bytes, max_batch=64)
+    sem = threading.Semaphore(max_inflight)
+    tickets = []
+    lock = threading.Lock()
+    done = threading.Event()
+    outs = {}
+
+    def waiter():
+        i = 0
+        while i < n_req:
+            with lock:
+                if i >= len(tickets):
+                    t = None
+                else:
+                    t = tickets[i]
+            if t is None:
+                _t.sleep(0.0002)
+                continue
+            srv.wait(t)
+            outs.pop(t, None)
+            sem.release()
+            i += 1
+        done.set()
+
+    th = threading.Thread(target=waiter)
+    th.start()
+    t0 = _t.perf_counter()
+    next_t = t0
+    for shape, pick in reqs:
+        if rate:
+            next_t += rng.expovariate(rate)
+            d = next_t - _t.perf_counter()
+            if d > 0:
+                _t.sleep(d)
+        sem.acquire()
+        x = pools[shape][pick]
+        out = torch.empty_like(x)
+        tk = srv.submit(x, out, *shape)
+        with lock:
+            outs[tk] = out
+            tickets.append(tk)
+    done.wait()
+    th.join()
+    st = srv.stats()
+    srv.close()
+    span = st["span_s"]
+    return {
+        "requests": n_req,
+        "offered_req_per_s": rate,
+        "achieved_req_per_s": round(n_req / span, 1) if span else None,
+        "gflops": round(st["nominal_gflop"] / span, 1) if span else None,
+        "p50_ms": round(st["p50_ms"], 3),
+        "p99_ms": round(st["p99_ms"], 3),
+        "max_ms": round(st["max_ms"], 3),
+        "batches": st["batches"],
+        "plans_created": st["plans_created"],
+        "peak_server_bytes": st["peak_bytes"],
+    }
+
+
 def cpu_baseline(sample_log2n):
     """The oracle's FMM-FFT (C restatement of src/fmmfft.c, OpenMP over the
     host cores) on N = 2^sample_log2n with the workload's P, order and leaf;
@@ -117,6 +210,9 @@ def main():
     ap.add_argument("--no-fp64", action="store_true", help="skip the complex128 comparison run")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-log2n", type=int, default=23)
+    ap.add_argument("--no-serve", action="store_true", help="skip the mixed request stream")
+    ap.add_argument("--serve-requests", type=int, default=3000,
+                    help="requests per serving phase (the full serving config uses 20000)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this FMM-FFT runs on one GPU; --gpus {args.gpus} is not supported "
@@ -190,6 +286,23 @@ def main():
         "roofline": roof,
     }
     line.update(extra)
+    if not args.no_serve:
+        del plan32, x32, out32
+        torch.cuda.empty_cache()
+        pools = {}
+        budget = 200 * (1 << 30)
+        sat = serving_stream(torch, fmm, args.serve_requests, None, 7, budget, pools)
+        sweep = []
+        if sat["achieved_req_per_s"]:
+            for frac in (0.5, 0.9):
+                sweep.append(serving_stream(torch, fmm, args.serve_requests,
+                                            frac * sat["achieved_req_per_s"], 11, budget, pools))
+        line["serving"] = {
+            "stream": "N=2^k, k~U{12..24}, P=2^min(6,k/4), p in {12,20}, 70% complex64 / 30% complex128, "
+                      "HBM budget 200 GB, max batch 64, device-resident inputs",
+            "saturation": sat,
+            "poisson": sweep,
+        }
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(args.cpu_sample_log2n)
     print(json.dumps(line), flush=True)

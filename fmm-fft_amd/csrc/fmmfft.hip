@@ -244,7 +244,12 @@ template <typename T, int L>
 void fft_len(const Geom &g, const FftPass &fp, unsigned nz, hipStream_t st, const cx<T> *in, cx<T> *out,
              int64_t in_stride, int64_t out_stride, const cx<T> *tw, const c64 *tlo, const c64 *thi) {
     constexpr int C = fft_cols<T, L>();
-    const unsigned nwg = (unsigned)((fp.F * g.P + C - 1) / C);
+    const int64_t ntiles = (fp.F * g.P + C - 1) / C;
+    // persistent grid: two resident workgroups per CU (LDS-limited)
+    int dev = 0, cus = 0;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    const unsigned nwg = (unsigned)std::min<int64_t>(ntiles, 2 * (int64_t)std::max(cus, 8));
     hipLaunchKernelGGL((k_fft<T, L>), dim3(nwg, 1, nz), dim3(256), 0, st, g, fp, in, out, in_stride, out_stride, tw,
                        tlo, thi);
 }
@@ -267,7 +272,11 @@ void launch_fft(int64_t L, const Geom &g, const FftPass &fp, unsigned nz, hipStr
 template <typename T>
 void launch_combine(const Geom &g, unsigned nz, hipStream_t st, const cx<T> *psec, int64_t ps, const c64 *sumsec,
                     int64_t ss, const c64 *gfac, const c64 *wP, cx<T> *y, int64_t ys) {
-    const dim3 grid((unsigned)((g.M + 31) / 32), 1, nz);
+    int dev = 0, cus = 0;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    const int64_t ntiles = (g.M + 31) / 32;
+    const dim3 grid((unsigned)std::min<int64_t>(ntiles, 4 * (int64_t)std::max(cus, 8)), 1, nz);
 #define COMBINE(PP) hipLaunchKernelGGL((k_combine<T, PP>), grid, dim3(256), 0, st, g, psec, ps, sumsec, ss, gfac, wP, y, ys)
     switch (g.P) {
         case 2: COMBINE(2); break;

@@ -172,9 +172,12 @@ __device__ inline int64_t xcd_remap(int64_t bid, int64_t nwg) {
     return (nwg % 8 == 0) ? (bid % 8) * (nwg / 8) + bid / 8 : bid;
 }
 
-// C sub-FFTs of length L per workgroup; twL = e^{-2 pi i k / L}, k < L.
-// Every lane first issues all of its global loads (PER elements, unrolled,
-// so they are in flight together), then fills LDS; the store side mirrors it.
+// C sub-FFTs of length L per tile; twL = e^{-2 pi i k / L}, k < L.
+// Persistent: a workgroup loops over tiles (signal = blockIdx.z) and issues
+// the global loads of its next tile into registers before transforming the
+// current one, so HBM latency hides behind the LDS work.  Each lane's output
+// elements share one sub-FFT f and step k by 256/C, so the pass-1 twiddle
+// w_M^{f k} is a per-lane FP64 recurrence started from two table lookups.
 template <typename T, int L, int C = fft_cols<T, L>()>
 __global__ __launch_bounds__(256) void k_fft(Geom g, FftPass fp, const cx<T> *__restrict__ in,
                                              cx<T> *__restrict__ out, int64_t in_stride, int64_t out_stride,
@@ -183,58 +186,85 @@ __global__ __launch_bounds__(256) void k_fft(Geom g, FftPass fp, const cx<T> *__
     constexpr int ls = row_stride<L>();
     constexpr int n = C * L;
     constexpr int PER = n / 256;
-    static_assert(n % 256 == 0, "a workgroup holds a multiple of 256 elements");
+    static_assert(n % 256 == 0, "a tile holds a multiple of 256 elements");
     __shared__ cx<T> a[C * ls];
     const int sig = blockIdx.z;
     in += (int64_t)sig * in_stride;
     out += (int64_t)sig * out_stride;
-    const int64_t fg0 = xcd_remap(blockIdx.x, gridDim.x) * C;  // first global sub-FFT
-    const int64_t nf = fp.F * g.P;                            // sub-FFTs per signal
-    const int cv = (int)min((int64_t)C, nf - fg0);            // valid sub-FFTs here
-    const int64_t k2 = fg0 / fp.F;
-    const int64_t f0 = fg0 - k2 * fp.F;
-    const int64_t base = k2 * g.M;
-    // when F == 1 the C sub-FFTs are C consecutive blocks
+    const int64_t nf = fp.F * g.P;  // sub-FFTs per signal
+    const int64_t ntiles = (nf + C - 1) / C;
+    const int64_t nwg = gridDim.x;
+    const int64_t xo = xcd_remap(blockIdx.x, nwg);
+    // when F == 1 the C sub-FFTs of a tile are C consecutive blocks
     const int64_t fstep_in = fp.F == 1 ? g.M : fp.in_fs;
     const int64_t fstep_out = fp.F == 1 ? g.M : fp.out_fs;
     const bool erow = fp.in_es == 1, krow = fp.out_ks == 1;
+    const int64_t Rm = (int64_t(1) << fp.logR) - 1;
 
     cx<T> v[PER];
+    auto prefetch = [&](int64_t t) {
+        const int64_t fg0 = t * C;
+        const int cv = (int)min((int64_t)C, nf - fg0);
+        const int64_t k2 = fg0 / fp.F, f0 = fg0 - k2 * fp.F, base = k2 * g.M;
 #pragma unroll
-    for (int u = 0; u < PER; ++u) {
-        const int i = threadIdx.x + u * 256;
-        const int f = erow ? i / L : i % C, e = erow ? i % L : i / C;
-        if (f < cv) v[u] = in[base + (f0 + f) * fstep_in + (int64_t)e * fp.in_es];
-    }
-#pragma unroll
-    for (int u = 0; u < PER; ++u) {
-        const int i = threadIdx.x + u * 256;
-        const int f = erow ? i / L : i % C, e = erow ? i % L : i / C;
-        a[f * ls + lpad(e)] = v[u];
-    }
-    __syncthreads();
-    stockham<T, L, C, 1>(a, ls, twL);
-#pragma unroll
-    for (int u = 0; u < PER; ++u) {
-        const int i = threadIdx.x + u * 256;
-        const int f = krow ? i / L : i % C, k = krow ? i % L : i / C;
-        v[u] = a[f * ls + lpad(k)];
-    }
-    const int64_t Rm = (int64_t(1) << fp.logR) - 1;
-#pragma unroll
-    for (int u = 0; u < PER; ++u) {
-        const int i = threadIdx.x + u * 256;
-        const int f = krow ? i / L : i % C, k = krow ? i % L : i / C;
-        if (f >= cv) continue;
-        cx<T> w = v[u];
-        if (fp.twiddle) {
-            const int64_t e = ((f0 + f) * (int64_t)k) & (g.M - 1);
-            const c64 t = cmul(thi[e >> fp.logR], tlo[e & Rm]);
-            const c64 r = cmul(c64{(double)w.re, (double)w.im}, t);
-            w = cx<T>{(T)r.re, (T)r.im};
+        for (int u = 0; u < PER; ++u) {
+            const int i = threadIdx.x + u * 256;
+            const int f = erow ? i / L : i % C, e = erow ? i % L : i / C;
+            if (f < cv) v[u] = in[base + (f0 + f) * fstep_in + (int64_t)e * fp.in_es];
         }
-        if (fp.conj_out) w.im = -w.im;
-        out[base + (f0 + f) * fstep_out + (int64_t)k * fp.out_ks] = w;
+    };
+
+    int64_t it = 0;
+    int64_t t = xo;
+    if (t < ntiles) prefetch(t);
+    while (t < ntiles) {
+        __syncthreads();  // the previous tile's outputs have left LDS
+#pragma unroll
+        for (int u = 0; u < PER; ++u) {
+            const int i = threadIdx.x + u * 256;
+            const int f = erow ? i / L : i % C, e = erow ? i % L : i / C;
+            a[f * ls + lpad(e)] = v[u];
+        }
+        __syncthreads();
+        const int64_t tn = (++it) * nwg + xo;
+        if (tn < ntiles) prefetch(tn);
+        stockham<T, L, C, 1>(a, ls, twL);
+
+        const int64_t fg0 = t * C;
+        const int cv = (int)min((int64_t)C, nf - fg0);
+        const int64_t k2 = fg0 / fp.F, f0 = fg0 - k2 * fp.F, base = k2 * g.M;
+        cx<T> o[PER];
+#pragma unroll
+        for (int u = 0; u < PER; ++u) {
+            const int i = threadIdx.x + u * 256;
+            const int f = krow ? i / L : i % C, k = krow ? i % L : i / C;
+            o[u] = a[f * ls + lpad(k)];
+        }
+        if (fp.twiddle) {
+            // pass 1 (krow false): lane's f = tid % C fixed, k = tid / C + u * (256 / C)
+            const int f = threadIdx.x % C;
+            const int64_t k0 = threadIdx.x / C;
+            const int64_t fa = f0 + f;
+            const int64_t e0 = (fa * k0) & (g.M - 1), es = (fa * (256 / C)) & (g.M - 1);
+            c64 w = cmul(thi[e0 >> fp.logR], tlo[e0 & Rm]);
+            const c64 ws = cmul(thi[es >> fp.logR], tlo[es & Rm]);
+#pragma unroll
+            for (int u = 0; u < PER; ++u) {
+                const c64 r = cmul(c64{(double)o[u].re, (double)o[u].im}, w);
+                o[u] = cx<T>{(T)r.re, (T)r.im};
+                w = cmul(w, ws);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < PER; ++u) {
+            const int i = threadIdx.x + u * 256;
+            const int f = krow ? i / L : i % C, k = krow ? i % L : i / C;
+            if (f >= cv) continue;
+            cx<T> w = o[u];
+            if (fp.conj_out) w.im = -w.im;
+            out[base + (f0 + f) * fstep_out + (int64_t)k * fp.out_ks] = w;
+        }
+        t = tn;
     }
 }
 

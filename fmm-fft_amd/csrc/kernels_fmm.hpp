@@ -656,8 +656,9 @@ __global__ __launch_bounds__(256) void k_leaf_mma(Geom g, const cx<T> *__restric
 //   y_k2[mu] = x_0[mu] + sum_sc w_P^{sc k2} gfac_sc (psec[sc][mu] - sumsec_sc),
 // i.e. the reference's dense fo product done as a P-point FFT (Stockham in
 // LDS, FP64 arithmetic for both precisions) and written block-major (the
-// reference's MPI_Alltoall).  32 rows per workgroup; reads and writes are
-// 32 contiguous elements per section / block, all issued before use.
+// reference's MPI_Alltoall).  Tiles of 32 rows; reads and writes are 32
+// contiguous elements per section / block.  Persistent workgroups prefetch
+// the next tile into registers while transforming the current one.
 // ---------------------------------------------------------------------------
 template <typename T, int P>
 __global__ __launch_bounds__(256) void k_combine(Geom g, const cx<T> *__restrict__ psec, int64_t psec_stride,
@@ -672,37 +673,63 @@ __global__ __launch_bounds__(256) void k_combine(Geom g, const cx<T> *__restrict
     psec += (int64_t)sig * psec_stride;
     sumsec += (int64_t)sig * sumsec_stride;
     y += (int64_t)sig * y_stride;
-    const int64_t mu0 = (int64_t)blockIdx.x * RB;
-    const int rv = (int)min((int64_t)RB, g.M - mu0);
+    const int64_t ntiles = (g.M + RB - 1) / RB;
+    const int64_t nwg = gridDim.x;
+    const int64_t xo = xcd_remap(blockIdx.x, nwg);
 
+    // per-lane section constants (a lane's sections are fixed across tiles)
+    c64 gf[PER], ss[PER];
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+        const int e = threadIdx.x + u * 256;
+        const int sc = e / RB;
+        gf[u] = (e < RB * P && sc != 0) ? gfac[sc] : c64{1.0, 0.0};
+        ss[u] = (e < RB * P && sc != 0) ? sumsec[sc] : c64{0.0, 0.0};
+    }
     cx<T> v[PER];
+    auto prefetch = [&](int64_t t) {
+        const int64_t mu0 = t * RB;
+        const int rv = (int)min((int64_t)RB, g.M - mu0);
 #pragma unroll
-    for (int u = 0; u < PER; ++u) {
-        const int e = threadIdx.x + u * 256;
-        const int sc = e / RB, r = e - sc * RB;
-        if (e < RB * P && r < rv) v[u] = psec[(int64_t)sc * g.M + mu0 + r];
-    }
-#pragma unroll
-    for (int u = 0; u < PER; ++u) {
-        const int e = threadIdx.x + u * 256;
-        const int sc = e / RB, r = e - sc * RB;
-        if (e >= RB * P) continue;
-        c64 w{(double)v[u].re, (double)v[u].im};
-        if (sc != 0) {
-            const c64 s = sumsec[sc];
-            w = cmul(gfac[sc], c64{w.re - s.re, w.im - s.im});
+        for (int u = 0; u < PER; ++u) {
+            const int e = threadIdx.x + u * 256;
+            const int sc = e / RB, r = e - sc * RB;
+            if (e < RB * P && r < rv) v[u] = psec[(int64_t)sc * g.M + mu0 + r];
         }
-        a[r * ls + lpad(sc)] = w;
-    }
-    __syncthreads();
-    stockham<double, P, RB, 1>(a, ls, wP);
+    };
+    int64_t it = 0, t = xo;
+    if (t < ntiles) prefetch(t);
+    while (t < ntiles) {
+        __syncthreads();  // the previous tile's outputs have left LDS
 #pragma unroll
-    for (int u = 0; u < PER; ++u) {
-        const int e = threadIdx.x + u * 256;
-        const int k2 = e / RB, r = e - k2 * RB;
-        if (e >= RB * P || r >= rv) continue;
-        const c64 w = a[r * ls + lpad(k2)];
-        y[(int64_t)k2 * g.M + mu0 + r] = cx<T>{(T)w.re, (T)w.im};
+        for (int u = 0; u < PER; ++u) {
+            const int e = threadIdx.x + u * 256;
+            const int sc = e / RB, r = e - sc * RB;
+            if (e >= RB * P) continue;
+            const c64 w{(double)v[u].re - ss[u].re, (double)v[u].im - ss[u].im};
+            a[r * ls + lpad(sc)] = cmul(gf[u], w);
+        }
+        __syncthreads();
+        const int64_t tn = (++it) * nwg + xo;
+        if (tn < ntiles) prefetch(tn);
+        stockham<double, P, RB, 1>(a, ls, wP);
+        const int64_t mu0 = t * RB;
+        const int rv = (int)min((int64_t)RB, g.M - mu0);
+        c64 o[PER];
+#pragma unroll
+        for (int u = 0; u < PER; ++u) {
+            const int e = threadIdx.x + u * 256;
+            const int k2 = e / RB, r = e - k2 * RB;
+            if (e < RB * P) o[u] = a[r * ls + lpad(k2)];
+        }
+#pragma unroll
+        for (int u = 0; u < PER; ++u) {
+            const int e = threadIdx.x + u * 256;
+            const int k2 = e / RB, r = e - k2 * RB;
+            if (e >= RB * P || r >= rv) continue;
+            y[(int64_t)k2 * g.M + mu0 + r] = cx<T>{(T)o[u].re, (T)o[u].im};
+        }
+        t = tn;
     }
 }
 
