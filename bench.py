@@ -202,6 +202,22 @@ def cpu_baseline(sample_log2n):
     }
 
 
+def measured_traffic(stage):
+    """HBM bytes per launch of one stage's kernel at this workload, as
+    measured by the rocprofv3 FETCH_SIZE and WRITE_SIZE passes summarised in
+    profiles/hbm_traffic_c64_n2e26.txt (each counter scaled by what it reads
+    for a plain copy of known size); None when that summary is absent."""
+    path = os.path.join(REPO, "profiles", "hbm_traffic_c64_n2e26.txt")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for ln in f:
+            kv = dict(t.split("=", 1) for t in ln.split() if "=" in t)
+            if kv.get("stage") == stage and "hbm_bytes" in kv:
+                return int(kv["hbm_bytes"])
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -213,6 +229,9 @@ def main():
     ap.add_argument("--no-serve", action="store_true", help="skip the mixed request stream")
     ap.add_argument("--serve-requests", type=int, default=3000,
                     help="requests per serving phase (the full serving config uses 20000)")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="after timing, copy the complex64 input twice with torch (N*8 bytes read and written "
+                         "per copy): the reference dispatch for scaling HBM byte counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this FMM-FFT runs on one GPU; --gpus {args.gpus} is not supported "
@@ -229,6 +248,10 @@ def main():
     out32 = torch.empty_like(x32)
     ms32, lat32, stages = time_plan(torch, plan32, x32, out32, args.steps, args.warmup, True)
     gflops32 = 5 * N * math.log2(N) / (ms32 * 1e-3) / 1e9
+    if args.calibrate_copy:
+        for _ in range(2):
+            out32.copy_(x32)
+        torch.cuda.synchronize()
 
     extra = {}
     if not args.no_fp64:
@@ -251,19 +274,20 @@ def main():
 
     per_stage = {k: v / stages[1] for k, v in stages[0].items()}
     dom = max(per_stage, key=per_stage.get)
+    traffic = measured_traffic(dom)
     M = N // P_BLOCKS
     S = P_BLOCKS - 1
     if dom == "leaf":
         work = 4.0 * S * M * (3 * LEAF + TERMS)  # complex x real MACs of near field + L2T
         achieved = work / (per_stage[dom] * 1e-3) / 1e12
         roof = {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": None, "kernel": "k_leaf<float>",
+                "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": traffic, "kernel": "k_leaf_mma<float>",
                 "algorithmic": f"{work:.4g} flop per launch"}
     else:
         by = 2.0 * N * 8  # one read + one write of the complex64 signal
         achieved = by / (per_stage[dom] * 1e-3) / 1e9
         roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None, "kernel": dom,
+                "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic, "kernel": dom,
                 "algorithmic": f"{by:.4g} bytes per launch"}
 
     line = {

@@ -240,18 +240,43 @@ TabPtrs<float> tab_ptrs<float>(const fmmfft_plan_s *p) {
 
 inline unsigned blocks_for(int64_t n, int per = 256) { return (unsigned)((n + per - 1) / per); }
 
+// Grid of a persistent kernel: every workgroup resident at once (compute
+// units x workgroups per unit from the occupancy calculator), so that the
+// strided tile loop never leaves a second, partial wave.
+unsigned resident_grid(const void *kern, int threads, size_t lds, int64_t ntiles) {
+    int dev = 0, cus = 0, per_cu = 0;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, threads, lds) != hipSuccess || per_cu < 1)
+        per_cu = 1;
+    return (unsigned)std::min<int64_t>(ntiles, (int64_t)std::max(cus, 8) * per_cu);
+}
+
 template <typename T, int L>
 void fft_len(const Geom &g, const FftPass &fp, unsigned nz, hipStream_t st, const cx<T> *in, cx<T> *out,
              int64_t in_stride, int64_t out_stride, const cx<T> *tw, const c64 *tlo, const c64 *thi) {
     constexpr int C = fft_cols<T, L>();
     const int64_t ntiles = (fp.F * g.P + C - 1) / C;
-    // persistent grid: two resident workgroups per CU (LDS-limited)
-    int dev = 0, cus = 0;
-    (void)hipGetDevice(&dev);
-    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    const unsigned nwg = (unsigned)std::min<int64_t>(ntiles, 2 * (int64_t)std::max(cus, 8));
-    hipLaunchKernelGGL((k_fft<T, L>), dim3(nwg, 1, nz), dim3(256), 0, st, g, fp, in, out, in_stride, out_stride, tw,
-                       tlo, thi);
+    const dim3 block(kFftThreads);
+    if (fp.in_es == 1 && fp.out_ks == 1) {
+        const void *k = (const void *)k_fft<T, L, FFT_SINGLE>;
+        const dim3 grid(resident_grid(k, kFftThreads, 0, ntiles), 1, nz);
+        hipLaunchKernelGGL((k_fft<T, L, FFT_SINGLE>), grid, block, 0, st, g, fp, in, out, in_stride, out_stride, tw,
+                           tlo, thi);
+    } else if (fp.in_es == 1) {
+        // FP64 pass 2 measures faster as one tile per workgroup (no
+        // persistence): its strided 64-byte output pieces gain from many
+        // workgroups in flight more than from the register prefetch
+        const void *k = (const void *)k_fft<T, L, FFT_PASS2>;
+        const dim3 grid(sizeof(T) == 8 ? (unsigned)ntiles : resident_grid(k, kFftThreads, 0, ntiles), 1, nz);
+        hipLaunchKernelGGL((k_fft<T, L, FFT_PASS2>), grid, block, 0, st, g, fp, in, out, in_stride, out_stride, tw,
+                           tlo, thi);
+    } else {
+        const void *k = (const void *)k_fft<T, L, FFT_PASS1>;
+        const dim3 grid(resident_grid(k, kFftThreads, 0, ntiles), 1, nz);
+        hipLaunchKernelGGL((k_fft<T, L, FFT_PASS1>), grid, block, 0, st, g, fp, in, out, in_stride, out_stride, tw,
+                           tlo, thi);
+    }
 }
 
 template <typename T>
@@ -272,12 +297,11 @@ void launch_fft(int64_t L, const Geom &g, const FftPass &fp, unsigned nz, hipStr
 template <typename T>
 void launch_combine(const Geom &g, unsigned nz, hipStream_t st, const cx<T> *psec, int64_t ps, const c64 *sumsec,
                     int64_t ss, const c64 *gfac, const c64 *wP, cx<T> *y, int64_t ys) {
-    int dev = 0, cus = 0;
-    (void)hipGetDevice(&dev);
-    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
     const int64_t ntiles = (g.M + 31) / 32;
-    const dim3 grid((unsigned)std::min<int64_t>(ntiles, 4 * (int64_t)std::max(cus, 8)), 1, nz);
-#define COMBINE(PP) hipLaunchKernelGGL((k_combine<T, PP>), grid, dim3(256), 0, st, g, psec, ps, sumsec, ss, gfac, wP, y, ys)
+#define COMBINE(PP)                                                                                          \
+    hipLaunchKernelGGL((k_combine<T, PP>),                                                                   \
+                       dim3(resident_grid((const void *)k_combine<T, PP>, kCombineThreads, 0, ntiles), 1, nz), \
+                       dim3(kCombineThreads), 0, st, g, psec, ps, sumsec, ss, gfac, wP, y, ys)
     switch (g.P) {
         case 2: COMBINE(2); break;
         case 4: COMBINE(4); break;

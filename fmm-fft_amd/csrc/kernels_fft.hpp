@@ -97,7 +97,13 @@ __device__ inline void dftR(cx<T> *v) {
     else dft16(v);
 }
 
+// Radix of the next Stockham stage for the remaining length rem.  FP64
+// stays at radix <= 8: a radix-16 butterfly with its 15 twiddles would take
+// ~190 registers per lane in FP64 and spill; the extra LDS round trip is
+// cheap next to the HBM time of a tile.
+template <typename T>
 constexpr int pick_radix(int rem) {
+    if (sizeof(T) == 8) return rem == 2 ? 2 : rem == 4 ? 4 : rem == 16 ? 4 : 8;
     return rem == 2 ? 2 : rem == 4 ? 4 : rem == 8 ? 8 : rem == 32 ? 8 : rem == 512 ? 8 : 16;
 }
 
@@ -105,17 +111,18 @@ constexpr int pick_radix(int rem) {
 // bank aliasing of the radix-16 stride.
 __device__ inline int lpad(int e) { return e + (e >> 4); }
 
-// One Stockham stage (current sub-transform size Ns, radix R) over C rows.
-template <typename T, int L, int C, int Ns, int R>
+// One Stockham stage (current sub-transform size Ns, radix R) over C rows,
+// NT threads.
+template <typename T, int L, int C, int Ns, int R, int NT>
 __device__ inline void stockham_stage(cx<T> *a, int ls, const cx<T> *__restrict__ twL) {
     constexpr int NB = L / R;
     constexpr int TOT = C * NB;
-    constexpr int BPT = (TOT + 255) / 256;
+    constexpr int BPT = (TOT + NT - 1) / NT;
     cx<T> v[BPT][R];
 #pragma unroll
     for (int bq = 0; bq < BPT; ++bq) {
-        const int w = threadIdx.x + bq * 256;
-        if (TOT % 256 == 0 || w < TOT) {
+        const int w = threadIdx.x + bq * NT;
+        if (TOT % NT == 0 || w < TOT) {
             const int f = w / NB, j = w % NB;
             const cx<T> *row = a + f * ls;
             const int k = j & (Ns - 1);
@@ -131,8 +138,8 @@ __device__ inline void stockham_stage(cx<T> *a, int ls, const cx<T> *__restrict_
     __syncthreads();
 #pragma unroll
     for (int bq = 0; bq < BPT; ++bq) {
-        const int w = threadIdx.x + bq * 256;
-        if (TOT % 256 == 0 || w < TOT) {
+        const int w = threadIdx.x + bq * NT;
+        if (TOT % NT == 0 || w < TOT) {
             const int f = w / NB, j = w % NB;
             cx<T> *row = a + f * ls;
             const int k = j & (Ns - 1);
@@ -144,12 +151,12 @@ __device__ inline void stockham_stage(cx<T> *a, int ls, const cx<T> *__restrict_
     __syncthreads();
 }
 
-template <typename T, int L, int C, int Ns>
+template <typename T, int L, int C, int Ns, int NT>
 __device__ inline void stockham(cx<T> *a, int ls, const cx<T> *__restrict__ twL) {
     if constexpr (Ns < L) {
-        constexpr int R = pick_radix(L / Ns);
-        stockham_stage<T, L, C, Ns, R>(a, ls, twL);
-        stockham<T, L, C, Ns * R>(a, ls, twL);
+        constexpr int R = pick_radix<T>(L / Ns);
+        stockham_stage<T, L, C, Ns, R, NT>(a, ls, twL);
+        stockham<T, L, C, Ns * R, NT>(a, ls, twL);
     }
 }
 
@@ -176,93 +183,99 @@ __device__ inline int64_t xcd_remap(int64_t bid, int64_t nwg) {
 // Persistent: a workgroup loops over tiles (signal = blockIdx.z) and issues
 // the global loads of its next tile into registers before transforming the
 // current one, so HBM latency hides behind the LDS work.  Each lane's output
-// elements share one sub-FFT f and step k by 256/C, so the pass-1 twiddle
+// elements share one sub-FFT f and step k by NT/C, so the pass-1 twiddle
 // w_M^{f k} is a per-lane FP64 recurrence started from two table lookups.
-template <typename T, int L, int C = fft_cols<T, L>()>
-__global__ __launch_bounds__(256) void k_fft(Geom g, FftPass fp, const cx<T> *__restrict__ in,
+// Offsets inside one signal fit 32 bits (N <= 2^26), which keeps the
+// per-element address arithmetic to one register.
+constexpr int kFftThreads = 512;
+
+// Access forms of a pass: element rows / output rows contiguous or strided.
+enum FftForm { FFT_SINGLE = 0, FFT_PASS1 = 1, FFT_PASS2 = 2 };
+
+template <typename T, int L, int FORM, int C = fft_cols<T, L>()>
+__global__ __launch_bounds__(kFftThreads) void k_fft(Geom g, FftPass fp, const cx<T> *__restrict__ in,
                                              cx<T> *__restrict__ out, int64_t in_stride, int64_t out_stride,
                                              const cx<T> *__restrict__ twL, const c64 *__restrict__ tlo,
                                              const c64 *__restrict__ thi) {
     constexpr int ls = row_stride<L>();
     constexpr int n = C * L;
-    constexpr int PER = n / 256;
-    static_assert(n % 256 == 0, "a tile holds a multiple of 256 elements");
+    constexpr int NT = kFftThreads;
+    constexpr int PER = n / NT;
+    static_assert(n % NT == 0 && NT % C == 0, "tile shape");
     __shared__ cx<T> a[C * ls];
     const int sig = blockIdx.z;
     in += (int64_t)sig * in_stride;
     out += (int64_t)sig * out_stride;
-    const int64_t nf = fp.F * g.P;  // sub-FFTs per signal
-    const int64_t ntiles = (nf + C - 1) / C;
-    const int64_t nwg = gridDim.x;
-    const int64_t xo = xcd_remap(blockIdx.x, nwg);
+    const int M = (int)g.M, F = (int)fp.F;
+    const int nf = F * (int)g.P;  // sub-FFTs per signal
+    const int ntiles = (nf + C - 1) / C;
+    const int nwg = gridDim.x;
+    const int xo = (int)xcd_remap(blockIdx.x, nwg);
     // when F == 1 the C sub-FFTs of a tile are C consecutive blocks
-    const int64_t fstep_in = fp.F == 1 ? g.M : fp.in_fs;
-    const int64_t fstep_out = fp.F == 1 ? g.M : fp.out_fs;
-    const bool erow = fp.in_es == 1, krow = fp.out_ks == 1;
-    const int64_t Rm = (int64_t(1) << fp.logR) - 1;
+    const int fstep_in = F == 1 ? M : (int)fp.in_fs;
+    const int fstep_out = F == 1 ? M : (int)fp.out_fs;
+    const int in_es = (int)fp.in_es, out_ks = (int)fp.out_ks;
+    // single pass: e and k run along rows; pass 1: both strided (columns of
+    // [M1][M2]); pass 2: e along rows, k strided
+    constexpr bool erow = FORM != FFT_PASS1, krow = FORM == FFT_SINGLE;
+    const int Rm = (1 << fp.logR) - 1;
 
-    cx<T> v[PER];
-    auto prefetch = [&](int64_t t) {
-        const int64_t fg0 = t * C;
-        const int cv = (int)min((int64_t)C, nf - fg0);
-        const int64_t k2 = fg0 / fp.F, f0 = fg0 - k2 * fp.F, base = k2 * g.M;
+    // re/im in separate scalar arrays and unconditional loads (a ragged last
+    // tile re-reads its last valid sub-FFT): the prefetch stays in registers.
+    T vr[PER], vi[PER];
+    auto prefetch = [&](int t) __attribute__((always_inline)) {
+        const int fg0 = t * C;
+        const int cl = min(C, nf - fg0) - 1;
+        const int k2 = fg0 / F, f0 = fg0 - k2 * F, base = k2 * M;
 #pragma unroll
         for (int u = 0; u < PER; ++u) {
-            const int i = threadIdx.x + u * 256;
+            const int i = threadIdx.x + u * NT;
             const int f = erow ? i / L : i % C, e = erow ? i % L : i / C;
-            if (f < cv) v[u] = in[base + (f0 + f) * fstep_in + (int64_t)e * fp.in_es];
+            const cx<T> x = in[base + (f0 + min(f, cl)) * fstep_in + e * in_es];
+            vr[u] = x.re;
+            vi[u] = x.im;
         }
     };
 
-    int64_t it = 0;
-    int64_t t = xo;
+    int t = xo;
     if (t < ntiles) prefetch(t);
     while (t < ntiles) {
         __syncthreads();  // the previous tile's outputs have left LDS
 #pragma unroll
         for (int u = 0; u < PER; ++u) {
-            const int i = threadIdx.x + u * 256;
+            const int i = threadIdx.x + u * NT;
             const int f = erow ? i / L : i % C, e = erow ? i % L : i / C;
-            a[f * ls + lpad(e)] = v[u];
+            a[f * ls + lpad(e)] = cx<T>{vr[u], vi[u]};
         }
         __syncthreads();
-        const int64_t tn = (++it) * nwg + xo;
+        const int tn = t + nwg;
         if (tn < ntiles) prefetch(tn);
-        stockham<T, L, C, 1>(a, ls, twL);
+        stockham<T, L, C, 1, NT>(a, ls, twL);
 
-        const int64_t fg0 = t * C;
-        const int cv = (int)min((int64_t)C, nf - fg0);
-        const int64_t k2 = fg0 / fp.F, f0 = fg0 - k2 * fp.F, base = k2 * g.M;
-        cx<T> o[PER];
+        const int fg0 = t * C;
+        const int cv = min(C, nf - fg0);
+        const int k2 = fg0 / F, f0 = fg0 - k2 * F, base = k2 * M;
+        c64 w{1.0, 0.0}, ws{1.0, 0.0};
+        if (fp.twiddle) {
+            // pass 1 (krow false): lane's f = tid % C fixed, k = tid / C + u * (NT / C)
+            const int fa = f0 + (int)threadIdx.x % C;
+            const int e0 = (int)(((int64_t)fa * (threadIdx.x / C)) & (M - 1));
+            const int es = (int)(((int64_t)fa * (NT / C)) & (M - 1));
+            w = cmul(thi[e0 >> fp.logR], tlo[e0 & Rm]);
+            ws = cmul(thi[es >> fp.logR], tlo[es & Rm]);
+        }
 #pragma unroll
         for (int u = 0; u < PER; ++u) {
-            const int i = threadIdx.x + u * 256;
+            const int i = threadIdx.x + u * NT;
             const int f = krow ? i / L : i % C, k = krow ? i % L : i / C;
-            o[u] = a[f * ls + lpad(k)];
-        }
-        if (fp.twiddle) {
-            // pass 1 (krow false): lane's f = tid % C fixed, k = tid / C + u * (256 / C)
-            const int f = threadIdx.x % C;
-            const int64_t k0 = threadIdx.x / C;
-            const int64_t fa = f0 + f;
-            const int64_t e0 = (fa * k0) & (g.M - 1), es = (fa * (256 / C)) & (g.M - 1);
-            c64 w = cmul(thi[e0 >> fp.logR], tlo[e0 & Rm]);
-            const c64 ws = cmul(thi[es >> fp.logR], tlo[es & Rm]);
-#pragma unroll
-            for (int u = 0; u < PER; ++u) {
-                const c64 r = cmul(c64{(double)o[u].re, (double)o[u].im}, w);
-                o[u] = cx<T>{(T)r.re, (T)r.im};
+            cx<T> val = a[f * ls + lpad(k)];
+            if (fp.twiddle) {
+                const c64 r = cmul(c64{(double)val.re, (double)val.im}, w);
+                val = cx<T>{(T)r.re, (T)r.im};
                 w = cmul(w, ws);
             }
-        }
-#pragma unroll
-        for (int u = 0; u < PER; ++u) {
-            const int i = threadIdx.x + u * 256;
-            const int f = krow ? i / L : i % C, k = krow ? i % L : i / C;
-            if (f >= cv) continue;
-            cx<T> w = o[u];
-            if (fp.conj_out) w.im = -w.im;
-            out[base + (f0 + f) * fstep_out + (int64_t)k * fp.out_ks] = w;
+            if (fp.conj_out) val.im = -val.im;
+            if (f < cv) out[base + (f0 + f) * fstep_out + k * out_ks] = val;
         }
         t = tn;
     }

@@ -459,7 +459,7 @@ __global__ __launch_bounds__(256) void k_leaf_mma(Geom g, const cx<T> *__restric
     T kvr[KPER], lr[LPER];
     // issue every global load of a tile into registers (they stay in flight
     // while the previous tile computes)
-    auto prefetch = [&](int lid) {
+    auto prefetch = [&](int lid) __attribute__((always_inline)) {
         const int sc0 = (lid % ngroups) * G, i0 = (lid / ngroups) * BB;
         const int64_t mu_w0 = (int64_t)(i0 - 2) * b;
 #pragma unroll
@@ -493,7 +493,7 @@ __global__ __launch_bounds__(256) void k_leaf_mma(Geom g, const cx<T> *__restric
             }
         }
     };
-    auto commit = [&]() {
+    auto commit = [&]() __attribute__((always_inline)) {
 #pragma unroll
         for (int u = 0; u < XPER; ++u) {
             const int e = threadIdx.x + 256 * u;
@@ -660,14 +660,17 @@ __global__ __launch_bounds__(256) void k_leaf_mma(Geom g, const cx<T> *__restric
 // contiguous elements per section / block.  Persistent workgroups prefetch
 // the next tile into registers while transforming the current one.
 // ---------------------------------------------------------------------------
+constexpr int kCombineThreads = 256;
+
 template <typename T, int P>
-__global__ __launch_bounds__(256) void k_combine(Geom g, const cx<T> *__restrict__ psec, int64_t psec_stride,
+__global__ __launch_bounds__(kCombineThreads) void k_combine(Geom g, const cx<T> *__restrict__ psec, int64_t psec_stride,
                                                  const c64 *__restrict__ sumsec, int64_t sumsec_stride,
                                                  const c64 *__restrict__ gfac, const c64 *__restrict__ wP,
                                                  cx<T> *__restrict__ y, int64_t y_stride) {
     constexpr int RB = 32;
+    constexpr int NT = kCombineThreads;
     constexpr int ls = row_stride<P>();
-    constexpr int PER = (RB * P + 255) / 256;
+    constexpr int PER = (RB * P + NT - 1) / NT;
     __shared__ c64 a[RB * ls];
     const int sig = blockIdx.z;
     psec += (int64_t)sig * psec_stride;
@@ -681,20 +684,24 @@ __global__ __launch_bounds__(256) void k_combine(Geom g, const cx<T> *__restrict
     c64 gf[PER], ss[PER];
 #pragma unroll
     for (int u = 0; u < PER; ++u) {
-        const int e = threadIdx.x + u * 256;
+        const int e = threadIdx.x + u * NT;
         const int sc = e / RB;
         gf[u] = (e < RB * P && sc != 0) ? gfac[sc] : c64{1.0, 0.0};
         ss[u] = (e < RB * P && sc != 0) ? sumsec[sc] : c64{0.0, 0.0};
     }
-    cx<T> v[PER];
-    auto prefetch = [&](int64_t t) {
+    // prefetch in scalar registers; a ragged last tile re-reads its last
+    // valid row (unconditional loads keep the arrays out of scratch)
+    T vr[PER], vi[PER];
+    auto prefetch = [&](int64_t t) __attribute__((always_inline)) {
         const int64_t mu0 = t * RB;
-        const int rv = (int)min((int64_t)RB, g.M - mu0);
+        const int rl = (int)min((int64_t)RB, g.M - mu0) - 1;
 #pragma unroll
         for (int u = 0; u < PER; ++u) {
-            const int e = threadIdx.x + u * 256;
-            const int sc = e / RB, r = e - sc * RB;
-            if (e < RB * P && r < rv) v[u] = psec[(int64_t)sc * g.M + mu0 + r];
+            const int e = threadIdx.x + u * NT;
+            const int sc = min(e / RB, P - 1), r = min(e % RB, rl);
+            const cx<T> x = psec[(int64_t)sc * g.M + mu0 + r];
+            vr[u] = x.re;
+            vi[u] = x.im;
         }
     };
     int64_t it = 0, t = xo;
@@ -703,31 +710,25 @@ __global__ __launch_bounds__(256) void k_combine(Geom g, const cx<T> *__restrict
         __syncthreads();  // the previous tile's outputs have left LDS
 #pragma unroll
         for (int u = 0; u < PER; ++u) {
-            const int e = threadIdx.x + u * 256;
+            const int e = threadIdx.x + u * NT;
             const int sc = e / RB, r = e - sc * RB;
             if (e >= RB * P) continue;
-            const c64 w{(double)v[u].re - ss[u].re, (double)v[u].im - ss[u].im};
+            const c64 w{(double)vr[u] - ss[u].re, (double)vi[u] - ss[u].im};
             a[r * ls + lpad(sc)] = cmul(gf[u], w);
         }
         __syncthreads();
         const int64_t tn = (++it) * nwg + xo;
         if (tn < ntiles) prefetch(tn);
-        stockham<double, P, RB, 1>(a, ls, wP);
+        stockham<double, P, RB, 1, NT>(a, ls, wP);
         const int64_t mu0 = t * RB;
         const int rv = (int)min((int64_t)RB, g.M - mu0);
-        c64 o[PER];
 #pragma unroll
         for (int u = 0; u < PER; ++u) {
-            const int e = threadIdx.x + u * 256;
-            const int k2 = e / RB, r = e - k2 * RB;
-            if (e < RB * P) o[u] = a[r * ls + lpad(k2)];
-        }
-#pragma unroll
-        for (int u = 0; u < PER; ++u) {
-            const int e = threadIdx.x + u * 256;
+            const int e = threadIdx.x + u * NT;
             const int k2 = e / RB, r = e - k2 * RB;
             if (e >= RB * P || r >= rv) continue;
-            y[(int64_t)k2 * g.M + mu0 + r] = cx<T>{(T)o[u].re, (T)o[u].im};
+            const c64 o = a[r * ls + lpad(k2)];
+            y[(int64_t)k2 * g.M + mu0 + r] = cx<T>{(T)o.re, (T)o.im};
         }
         t = tn;
     }
