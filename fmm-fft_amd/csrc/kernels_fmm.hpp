@@ -178,7 +178,9 @@ __global__ __launch_bounds__(256) void k_tree(Geom g, int l, int mode, const T *
     __syncthreads();
 
     const int nn = 1 << l;
-    const int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    // consecutive workgroups (neighbouring nodes, which share the M2L
+    // sources c +- 2, 3) run on one XCD and so share its L2
+    const int64_t o = xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (o >= (int64_t)nn * rows) return;
     const int c = (int)(o / rows), row = (int)(o - (int64_t)c * rows);
     const int64_t ts = (int64_t)blockIdx.z * tree_stride;

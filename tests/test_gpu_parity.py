@@ -265,3 +265,42 @@ def test_fp32_batched_form(fmm, orc, torch):
         xi = xd[i].cpu().numpy().astype(np.complex128)
         err = rel(X[i].cpu().numpy().astype(np.complex128), orc.fft64(xi))
         assert err < 1e-6, (i, err)
+
+
+# ---------------------------------------------------------------------------
+# the reference's own recorded end-to-end errors (ramp input, b=32, T=15)
+# ---------------------------------------------------------------------------
+
+def _reference_rows():
+    import csv
+    import os
+
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "reference_max_error.csv")
+    with open(path) as f:
+        rows = list(csv.DictReader(f))
+    return [(int(r["N"]), int(r["P"]), int(r["B"]), int(r["T"]), r["max_rel_err_pct"])
+            for r in rows if int(r["N"]) <= (1 << 22) and int(r["N"]) // int(r["P"]) <= (1 << 20)]
+
+
+def ramp_dft(n, m):
+    """Exact DFT of x_j = j + 1 at k < m: Y_0 = n(n+1)/2, Y_k = n / (e^{-2 pi i k/n} - 1)."""
+    k = np.arange(m, dtype=np.float64)
+    a = 2 * np.pi * k / n
+    den = -2.0 * np.sin(a / 2) ** 2 - 1j * np.sin(a)
+    y = np.empty(m, np.complex128)
+    y[1:] = n / den[1:]
+    y[0] = n * (n + 1) / 2
+    return y
+
+
+@pytest.mark.parametrize("N,P,B,T,pct", _reference_rows())
+def test_reference_recorded_max_error(fmm, torch, N, P, B, T, pct):
+    """src/fmmfft_error.c:1097-1108,1267 printed max_k<M |X_k - Y_k|/|Y_k|*100
+    for the ramp; the complex128 GPU path must print the same '%lf'."""
+    say(f"reference row N={N} P={P} B={B} T={T}")
+    plan = fmm.Plan(N, P, T, B, fmm.C128)
+    x = torch.arange(1, N + 1, dtype=torch.float64, device="cuda").to(torch.complex128)
+    M = N // P
+    X = plan.execute(x)[:M].cpu().numpy()
+    Y = ramp_dft(N, M)
+    assert "%f" % float(np.max(np.abs(X - Y) / np.abs(Y)) * 100) == pct
