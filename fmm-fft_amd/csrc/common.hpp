@@ -53,6 +53,11 @@ struct Geom {
     int s, n;       // leaf boxes per section (2^n), tree depth
 };
 
+// L2T K steps (of 4 terms) per row tile of the matrix-core leaf kernel; the
+// evaluation-row table and the kernel's LDS copy of the local expansions are
+// padded to 4 * leaf_kt(terms) terms.
+__host__ __device__ inline int leaf_kt(int terms) { return terms <= 16 ? 4 : 8; }
+
 // Real offset of level l (levels 2..n, 2^l nodes of [terms][S][2] reals).
 __host__ __device__ inline int64_t level_off(const Geom &g, int l) {
     return ((int64_t(1) << l) - 4) * int64_t(g.S) * 2 * g.terms;

@@ -112,7 +112,7 @@ TableLayout table_layout(const Geom &g, const FftLayout &f) {
     t.evalm = o; o += (size_t)g.S * g.b * T;
     t.evalmh = o; o += (size_t)(g.P / 2) * T;
     t.kv = o; o += (size_t)g.S * 4 * g.b;
-    t.evfrag = o; o += (g.b % 16 == 0) ? (size_t)g.S * (g.b / 16) * ((g.terms + 3) / 4) * 64 : 0;
+    t.evfrag = o; o += (g.b % 16 == 0) ? (size_t)g.S * (g.b / 16) * leaf_kt(g.terms) * 64 : 0;
     t.n_ops = o;
     o = (o + 1) & ~size_t(1);
     t.gfac = o; o += 2 * (size_t)g.P;
@@ -145,7 +145,7 @@ size_t leaf_lds(const Geom &g, size_t esize, int G, int bb) {
 
 size_t leaf_mma_lds(const Geom &g, size_t esize, int G) {
     return ((size_t)11 * g.b * G + 11) * 2 * esize + (size_t)G * 4 * g.b * esize +
-           (size_t)9 * (g.terms * 2 * G + 2) * esize;
+           (size_t)9 * (4 * leaf_kt(g.terms) * 2 * G + 2) * esize;
 }
 // (the persistent leaf kernel's LDS is at most 11*128*4*16 + ... < 100 KB)
 
@@ -334,7 +334,7 @@ void leaf_mma_inst(const Geom &g, const Shapes &sh, unsigned nz, hipStream_t st,
 template <typename T, int NRT, int G>
 void leaf_mma_kt(const Geom &g, const Shapes &sh, unsigned nz, hipStream_t st, const cx<T> *x, const T *psi,
                  int64_t wstride_T, cx<T> *psec, int64_t wstride_cx, const TabPtrs<T> &tb, int conj) {
-    if (g.terms <= 16)
+    if (leaf_kt(g.terms) == 4)
         leaf_mma_inst<T, NRT, G, 4>(g, sh, nz, st, x, psi, wstride_T, psec, wstride_cx, tb, conj);
     else
         leaf_mma_inst<T, NRT, G, 8>(g, sh, nz, st, x, psi, wstride_T, psec, wstride_cx, tb, conj);
@@ -549,7 +549,7 @@ int fmmfft_plan_create(fmmfft_plan *out, int64_t n, int32_t p_blocks, int32_t te
     hipLaunchKernelGGL(k_build_interp, dim3(jobs), dim3(64), 0, st, g, t + tl.ops, t + tl.topflp,
                        t + tl.evalm, t + tl.evalmh);
     if (g.b % 16 == 0) {
-        const int nrt = g.b / 16, kt = (g.terms + 3) / 4;
+        const int nrt = g.b / 16, kt = leaf_kt(g.terms);
         hipLaunchKernelGGL(k_build_evfrag, dim3(blocks_for((int64_t)g.S * nrt * kt * 64)), dim3(256), 0, st, g,
                            (const double *)(t + tl.evalm), t + tl.evfrag, nrt, kt);
     }

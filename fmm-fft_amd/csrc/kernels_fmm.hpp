@@ -400,6 +400,18 @@ __global__ __launch_bounds__(256) void k_leaf(Geom g, const cx<T> *__restrict__ 
 // and receives D[row][l%16] for its four accumulators r = 0..3, with
 // row = 4*(l/16) + r for FP32 and l/16 + 4*r for FP64.
 // ---------------------------------------------------------------------------
+// p ? a : b as a bit select (keeps the compiler from folding selects between
+// vector elements into a dynamically indexed extract)
+__device__ inline float bsel(int p, float a, float b) {
+    const unsigned m = 0u - (unsigned)(p != 0);
+    return __uint_as_float((__float_as_uint(a) & m) | (__float_as_uint(b) & ~m));
+}
+__device__ inline double bsel(int p, double a, double b) {
+    const unsigned long long m = 0ull - (unsigned long long)(p != 0);
+    return __longlong_as_double((long long)(((unsigned long long)__double_as_longlong(a) & m) |
+                                            ((unsigned long long)__double_as_longlong(b) & ~m)));
+}
+
 template <typename T>
 struct Mfma;
 // Output row of accumulator element r held by lanes with l/16 = kr (both
@@ -433,17 +445,23 @@ __global__ __launch_bounds__(256) void k_leaf_mma(Geom g, const cx<T> *__restric
     constexpr int ROWS = (BB + 3) * b;
     constexpr int XN = ROWS * G, XPER = (XN + 255) / 256;           // x window
     constexpr int KVN = G * 4 * b, KPER = (KVN + 255) / 256;        // kernel samples
-    constexpr int LMAX = (BB + 1) * 4 * KT * 2 * G, LPER = (LMAX + 255) / 256;  // local expansions
+    // local expansions in LDS with the terms padded to TP = 4 KT (zeros past
+    // terms), so every index below is a compile-time shift or mask
+    constexpr int TP = 4 * KT;
+    constexpr int NL = TP * 2 * G;                                   // one box
+    constexpr int LSTR = NL + 2;                                     // +2: bank shift
+    constexpr int LN = (BB + 1) * NL, LPER = (LN + 255) / 256;
     extern __shared__ __align__(16) unsigned char smem[];
     const int P = g.P, terms = g.terms, S = g.S;
     const int64_t M = g.M;
     const int rowsS = 2 * S;             // tree row stride
     const int kt = (terms + 3) / 4;      // L2T K steps (<= KT)
-    const int nloc = terms * 2 * G;
-    const int lstr = nloc + 2;           // LDS stride of one box's local expansions (+2: bank shift)
+    // offsets inside one signal fit 32 bits (N <= 2^26); P, M and s are
+    // powers of two
+    const int lp = __builtin_ctz(P), mmask = (int)M - 1, smask = g.s - 1;
     cx<T> *xw = reinterpret_cast<cx<T> *>(smem);           // [ROWS][G] + 1 pad per box
     T *kvs = reinterpret_cast<T *>(xw + XN + BB + 3);      // [G][4b]
-    T *lsx = kvs + KVN;                                    // [BB+1][terms][G][2] (+2 per box)
+    T *lsx = kvs + KVN;                                    // [BB+1][TP][G][2] (+2 per box)
     const int sig = blockIdx.z;
     x += (int64_t)sig * x_stride;
     psi += (int64_t)sig * tree_stride + level_off(g, g.n);
@@ -463,15 +481,14 @@ __global__ __launch_bounds__(256) void k_leaf_mma(Geom g, const cx<T> *__restric
     // while the previous tile computes)
     auto prefetch = [&](int lid) __attribute__((always_inline)) {
         const int sc0 = (lid % ngroups) * G, i0 = (lid / ngroups) * BB;
-        const int64_t mu_w0 = (int64_t)(i0 - 2) * b;
+        const int mu_w0 = (i0 - 2) * b;
 #pragma unroll
         for (int u = 0; u < XPER; ++u) {
             const int e = threadIdx.x + 256 * u;
             if (XN % 256 == 0 || e < XN) {
                 const int r = e / G, gg = e % G;
-                int64_t mu = mu_w0 + r;
-                mu = mu < 0 ? mu + M : (mu >= M ? mu - M : mu);
-                xr[u] = x[mu * P + sc0 + gg];
+                const int mu = (mu_w0 + r) & mmask;  // periodic window
+                xr[u] = x[(mu << lp) + sc0 + gg];
             }
         }
 #pragma unroll
@@ -480,18 +497,21 @@ __global__ __launch_bounds__(256) void k_leaf_mma(Geom g, const cx<T> *__restric
             if (KVN % 256 == 0 || e < KVN) {
                 const int gg = e / (4 * b), d = e % (4 * b);
                 const int sc = sc0 + gg;
-                kvr[u] = sc == 0 ? T(0) : kv[(size_t)(sc - 1) * 4 * b + d];
+                const T v = kv[max(sc - 1, 0) * 4 * b + d];
+                kvr[u] = sc == 0 ? T(0) : v;
             }
         }
 #pragma unroll
         for (int u = 0; u < LPER; ++u) {
             const int e = threadIdx.x + 256 * u;
-            if (e < (BB + 1) * nloc) {
-                const int bx9 = e / nloc, rem = e - bx9 * nloc;
-                const int k = rem / (2 * G), rr = rem - k * 2 * G;
+            if (LN % 256 == 0 || e < LN) {
+                const int bx9 = e / NL, rem = e % NL;
+                const int k = rem / (2 * G), rr = rem % (2 * G);
                 const int sc = sc0 + (rr >> 1);
-                const int box = pmod(i0 - 1 + bx9, g.s);
-                lr[u] = sc == 0 ? T(0) : psi[((int64_t)box * terms + k) * rowsS + 2 * (sc - 1) + (rr & 1)];
+                const int box = (i0 - 1 + bx9) & smask;
+                // clamped, unconditional load; padding terms and section 0 read as 0
+                const T v = psi[(box * terms + min(k, terms - 1)) * rowsS + max(2 * (sc - 1) + (rr & 1), 0)];
+                lr[u] = (k < terms && sc != 0) ? v : T(0);
             }
         }
     };
@@ -513,10 +533,7 @@ __global__ __launch_bounds__(256) void k_leaf_mma(Geom g, const cx<T> *__restric
 #pragma unroll
         for (int u = 0; u < LPER; ++u) {
             const int e = threadIdx.x + 256 * u;
-            if (e < (BB + 1) * nloc) {
-                const int bx9 = e / nloc;
-                lsx[bx9 * lstr + (e - bx9 * nloc)] = lr[u];
-            }
+            if (LN % 256 == 0 || e < LN) lsx[(e / NL) * LSTR + e % NL] = lr[u];
         }
     };
 
@@ -544,13 +561,14 @@ __global__ __launch_bounds__(256) void k_leaf_mma(Geom g, const cx<T> *__restric
                 continue;
             }
             const T *kr_ = kvs + gg * 4 * b;
-            // L2T A fragments, stored in lane order: one coalesced load per step
+            // L2T A fragments, stored in lane order with KT steps per row tile
+            // (zero past terms): one coalesced load per step, constant offsets
             T la[NRT][KT];
+            const T *ef = evfrag + (((sc - 1) * NRT * KT) << 6) + lane;
 #pragma unroll
             for (int s2 = 0; s2 < KT; ++s2)
 #pragma unroll
-                for (int rt = 0; rt < NRT; ++rt)
-                    la[rt][s2] = s2 < kt ? evfrag[((((size_t)(sc - 1) * NRT + rt) * kt + s2) << 6) + lane] : T(0);
+                for (int rt = 0; rt < NRT; ++rt) la[rt][s2] = ef[(rt * KT + s2) << 6];
             V4 acc[NRT];
 #pragma unroll
             for (int rt = 0; rt < NRT; ++rt) acc[rt] = V4{0, 0, 0, 0};
@@ -603,12 +621,11 @@ __global__ __launch_bounds__(256) void k_leaf_mma(Geom g, const cx<T> *__restric
                 }
             }
             // far field (L2T): B = local expansion of box bx from LDS
-            const T *lb = lsx + (bx + 1) * lstr + 2 * gg + c;
+            const T *lb = lsx + (bx + 1) * LSTR + 2 * gg + c;
 #pragma unroll
             for (int s2 = 0; s2 < KT; ++s2)
                 if (s2 < kt) {
-                    const int kk = 4 * s2 + kr;
-                    const T bvl = kk < terms ? lb[kk * 2 * G] : T(0);
+                    const T bvl = lb[(4 * s2 + kr) * 2 * G];
 #pragma unroll
                     for (int rt = 0; rt < NRT; ++rt) acc[rt] = Mfma<T>::mma(la[rt][s2], bvl, acc[rt]);
                 }
@@ -620,7 +637,7 @@ __global__ __launch_bounds__(256) void k_leaf_mma(Geom g, const cx<T> *__restric
                 const T *xp1 = xwc + ((size_t)(bx + 3) * b * G + gg + bx + 3) * 2 + c;  // box i+1
                 for (int t = kr; t < b; t += 4)
                     corr += kr_[t] * xm2[(size_t)t * G * 2] - kr_[3 * b + t] * xp1[(size_t)t * G * 2];
-                const T *lm = lsx + bx * lstr + 2 * gg + c;  // box i-1
+                const T *lm = lsx + bx * LSTR + 2 * gg + c;  // box i-1
                 const T *eh = evalmh + (size_t)(sc - P / 2) * terms;
                 const T *e0 = evalm + (size_t)(sc - 1) * b * terms;  // row of target 0
                 for (int kk = kr; kk < terms; kk += 4) corr += eh[kk] * lm[kk * 2 * G] - e0[kk] * lb[kk * 2 * G];
@@ -630,15 +647,17 @@ __global__ __launch_bounds__(256) void k_leaf_mma(Geom g, const cx<T> *__restric
             }
             // write: pair lanes (c = 0, 1) swap halves so that each lane holds
             // two complete complex outputs; 8 lanes cover 16 consecutive targets
+            // (bit selects: a ?: between vector elements becomes an indexed
+            // extract over all accumulators, a long compare/select chain)
             cx<T> *dst = psec + (int64_t)sc * M + (int64_t)(i0 + bx) * b;
 #pragma unroll
             for (int rt = 0; rt < NRT; ++rt) {
-                T send0 = c ? acc[rt][0] : acc[rt][2];
-                T send1 = c ? acc[rt][1] : acc[rt][3];
-                const T recv0 = __shfl_xor(send0, 1, 64);
-                const T recv1 = __shfl_xor(send1, 1, 64);
-                const cx<T> o0 = c ? cx<T>{recv0, acc[rt][2]} : cx<T>{acc[rt][0], recv0};
-                const cx<T> o1 = c ? cx<T>{recv1, acc[rt][3]} : cx<T>{acc[rt][1], recv1};
+                const T a0 = acc[rt][0], a1 = acc[rt][1], a2 = acc[rt][2], a3 = acc[rt][3];
+                const T keep0 = bsel(c, a2, a0), keep1 = bsel(c, a3, a1);
+                const T recv0 = __shfl_xor(bsel(c, a0, a2), 1, 64);
+                const T recv1 = __shfl_xor(bsel(c, a1, a3), 1, 64);
+                const cx<T> o0{bsel(c, recv0, keep0), bsel(c, keep0, recv0)};
+                const cx<T> o1{bsel(c, recv1, keep1), bsel(c, keep1, recv1)};
                 if constexpr (sizeof(T) == 4) {
                     // rows 4 kr + 2c, 4 kr + 2c + 1: one 16-byte store
                     *reinterpret_cast<float4 *>(dst + 16 * rt + Mfma<T>::row(kr, 2 * c)) =
