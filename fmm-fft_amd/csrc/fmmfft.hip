@@ -73,7 +73,7 @@ int validate(int64_t n, int32_t P, int32_t terms, int32_t leaf, int32_t dtype, P
 // Offsets (in doubles) of the FP64 operator tables; the FP32 copies use the
 // same offsets in floats for the first `n_ops` entries.
 struct TableLayout {
-    size_t cheb, initch, ops, topflp, evalm, evalmh, kv, evfrag, n_ops;  // contiguous block
+    size_t cheb, initch, ops, topflp, evalm, evalmh, kv, evfrag, opfrag, n_ops;  // contiguous block
     size_t gfac, wP;                                             // complex128 [P]
     size_t twA, twB, tlo, thi;                                   // complex128
     size_t total;                                                // doubles
@@ -113,6 +113,8 @@ TableLayout table_layout(const Geom &g, const FftLayout &f) {
     t.evalmh = o; o += (size_t)(g.P / 2) * T;
     t.kv = o; o += (size_t)g.S * 4 * g.b;
     t.evfrag = o; o += (g.b % 16 == 0) ? (size_t)g.S * (g.b / 16) * leaf_kt(g.terms) * 64 : 0;
+    const size_t tp = 4 * (size_t)leaf_kt(g.terms);
+    t.opfrag = o; o += ((size_t)(g.n - 2) * NOPS + 1) * tp * tp;
     t.n_ops = o;
     o = (o + 1) & ~size_t(1);
     t.gfac = o; o += 2 * (size_t)g.P;
@@ -219,7 +221,7 @@ namespace {
 
 template <typename T>
 struct TabPtrs {
-    const T *initch, *ops, *topflp, *evalm, *evalmh, *kv, *evfrag, *twA, *twB;
+    const T *initch, *ops, *topflp, *evalm, *evalmh, *kv, *evfrag, *opfrag, *twA, *twB;
 };
 
 template <typename T>
@@ -229,13 +231,13 @@ template <>
 TabPtrs<double> tab_ptrs<double>(const fmmfft_plan_s *p) {
     const double *t = p->tab64;
     return {t + p->tl.initch, t + p->tl.ops, t + p->tl.topflp, t + p->tl.evalm, t + p->tl.evalmh,
-            t + p->tl.kv,     t + p->tl.evfrag, t + p->tl.twA, t + p->tl.twB};
+            t + p->tl.kv,     t + p->tl.evfrag, t + p->tl.opfrag, t + p->tl.twA, t + p->tl.twB};
 }
 template <>
 TabPtrs<float> tab_ptrs<float>(const fmmfft_plan_s *p) {
     const float *t = p->tab32;
     return {t + p->tl.initch, t + p->tl.ops, t + p->tl.topflp, t + p->tl.evalm, t + p->tl.evalmh,
-            t + p->tl.kv,     t + p->tl.evfrag, t + p->tw32_A, t + p->tw32_B};
+            t + p->tl.kv,     t + p->tl.evfrag, t + p->tl.opfrag, t + p->tw32_A, t + p->tw32_B};
 }
 
 inline unsigned blocks_for(int64_t n, int per = 256) { return (unsigned)((n + per - 1) / per); }
@@ -369,15 +371,24 @@ void s2m_tree_pt(const Geom &g, const Shapes &sh, unsigned nz, hipStream_t st, c
     hipLaunchKernelGGL(k_sums_reduce, dim3(g.P, 1, nz), dim3(256), 0, st, g, sums_part, sh.s2m_nblk, wstride_c64,
                        sumsec, wstride_c64);
     mark(1);
-    const int64_t rows = 2 * (int64_t)g.S;
+    // tree passes on the matrix cores: terms padded to 16 or 32, one wave per
+    // (node, block of 16 rows), four waves per workgroup
+    constexpr int TP = PT <= 16 ? 16 : 32;
+    const int64_t nrb = (2 * (int64_t)g.S + 15) / 16;
+    auto grid = [&](int l, int mode) {
+        const int rb = tree_mma_rb<T, TP>(mode);
+        const int64_t waves = (int64_t(1) << l) * ((nrb + rb - 1) / rb);
+        return dim3((unsigned)((waves + 3) / 4), 1, nz);
+    };
     for (int l = g.n - 1; l >= 2; --l)
-        hipLaunchKernelGGL((k_tree<T, PT>), dim3(blocks_for((int64_t(1) << l) * rows), 1, nz), dim3(256), 0, st, g,
-                           l, 0, (const T *)phi, phi, (const T *)psi, psi, wstride_T, tb.ops, tb.topflp);
+        hipLaunchKernelGGL((k_tree_mma<T, TP, 0>), grid(l, 0), dim3(256), 0, st, g, l, (const T *)phi, phi,
+                           (const T *)psi, psi, wstride_T, tb.opfrag);
     mark(2);
-    for (int l = 2; l <= g.n; ++l)
-        hipLaunchKernelGGL((k_tree<T, PT>), dim3(blocks_for((int64_t(1) << l) * rows), 1, nz), dim3(256), 0, st, g,
-                           l, l == 2 ? 2 : 1, (const T *)phi, phi, (const T *)psi, psi, wstride_T, tb.ops,
-                           tb.topflp);
+    hipLaunchKernelGGL((k_tree_mma<T, TP, 2>), grid(2, 2), dim3(256), 0, st, g, 2, (const T *)phi, phi,
+                       (const T *)psi, psi, wstride_T, tb.opfrag);
+    for (int l = 3; l <= g.n; ++l)
+        hipLaunchKernelGGL((k_tree_mma<T, TP, 1>), grid(l, 1), dim3(256), 0, st, g, l, (const T *)phi, phi,
+                           (const T *)psi, psi, wstride_T, tb.opfrag);
 }
 
 template <typename T, typename Mark>
@@ -552,6 +563,12 @@ int fmmfft_plan_create(fmmfft_plan *out, int64_t n, int32_t p_blocks, int32_t te
         const int nrt = g.b / 16, kt = leaf_kt(g.terms);
         hipLaunchKernelGGL(k_build_evfrag, dim3(blocks_for((int64_t)g.S * nrt * kt * 64)), dim3(256), 0, st, g,
                            (const double *)(t + tl.evalm), t + tl.evfrag, nrt, kt);
+    }
+    {
+        const int tp = 4 * leaf_kt(g.terms);
+        const int64_t nf = ((int64_t)(g.n - 2) * NOPS + 1) * tp * tp;
+        hipLaunchKernelGGL(k_build_opfrag, dim3(blocks_for(nf)), dim3(256), 0, st, g, (const double *)(t + tl.ops),
+                           (const double *)(t + tl.topflp), t + tl.opfrag, tp);
     }
     const int64_t R = int64_t(1) << p->fft.logR;
     const int64_t MA = p->fft.M1, MB = p->fft.M2;

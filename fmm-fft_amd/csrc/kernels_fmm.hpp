@@ -127,123 +127,6 @@ __global__ __launch_bounds__(256) void k_sums_reduce(Geom g, const c64 *__restri
 }
 
 // ---------------------------------------------------------------------------
-// One tree level, one lane per output row (node, section, re/im): the lane
-// forms all `terms` outputs as a sum of up to four p x p operators applied to
-// input rows, with the operators staged in LDS (every lane of a wave reads
-// the same entry: broadcast) and the rows read term by term (lanes read
-// consecutive rows: coalesced).
-//   mode 0, upward, parent level l (src/fmmfft.c:476-515, mpp :174-184):
-//     phi_l[q] = shftfn . phi_{l+1}[2q] + shftfp . phi_{l+1}[2q+1]
-//   mode 1, downward, level l >= 3 (src/fmmfft.c:644-749):
-//     even c: shftlp . psi_{l-1}[c/2] + flip2p phi_l[c-2] + flip2n phi_l[c+2] + flip3n phi_l[c+3]
-//     odd  c: shftln . psi_{l-1}[c/2] + flip3p phi_l[c-3] + flip2p phi_l[c-2] + flip2n phi_l[c+2]
-//     node indices periodic (the reference's ring of ranks)
-//   mode 2, top flip (src/fmmfft.c:620-639): psi_2[q] = topflp . phi_2[q+2 mod 4]
-// ---------------------------------------------------------------------------
-template <typename T, int PT>
-__global__ __launch_bounds__(256) void k_tree(Geom g, int l, int mode, const T *__restrict__ phi_in,
-                                              T *__restrict__ phi_out, const T *__restrict__ psi_in,
-                                              T *__restrict__ psi_out, int64_t tree_stride,
-                                              const T *__restrict__ ops, const T *__restrict__ topflp) {
-    __shared__ T As[6 * PT * PT];
-    const int terms = g.terms, rows = 2 * g.S, tt = terms * terms;
-    const int node_len = rows * terms;
-    // stage the operators, zero padded to PT x PT
-    int nmat;
-    const T *mats[6];
-    if (mode == 0) {
-        const T *o = ops + (size_t)(l - 2) * NOPS * tt;
-        nmat = 2;
-        mats[0] = o + OP_SHFTFN * tt;
-        mats[1] = o + OP_SHFTFP * tt;
-    } else if (mode == 1) {
-        const T *o = ops + (size_t)(l - 3) * NOPS * tt;
-        nmat = 6;
-        mats[0] = o + OP_SHFTLP * tt;
-        mats[1] = o + OP_SHFTLN * tt;
-        mats[2] = o + OP_FLIP2P * tt;
-        mats[3] = o + OP_FLIP2N * tt;
-        mats[4] = o + OP_FLIP3N * tt;
-        mats[5] = o + OP_FLIP3P * tt;
-    } else {
-        nmat = 1;
-        mats[0] = topflp;
-    }
-    for (int i = threadIdx.x; i < nmat * PT * PT; i += blockDim.x) {
-        const int m = i / (PT * PT), r = i - m * PT * PT;
-        const int k = r / PT, kk = r - k * PT;
-        const T *src = m == 0 ? mats[0] : m == 1 ? mats[1] : m == 2 ? mats[2] : m == 3 ? mats[3] : m == 4 ? mats[4] : mats[5];
-        As[i] = (k < terms && kk < terms) ? src[k * terms + kk] : T(0);
-    }
-    __syncthreads();
-
-    const int nn = 1 << l;
-    // consecutive workgroups (neighbouring nodes, which share the M2L
-    // sources c +- 2, 3) run on one XCD and so share its L2
-    const int64_t o = xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    if (o >= (int64_t)nn * rows) return;
-    const int c = (int)(o / rows), row = (int)(o - (int64_t)c * rows);
-    const int64_t ts = (int64_t)blockIdx.z * tree_stride;
-
-    const T *src[4];
-    int mi[4];
-    int nt;
-    T *dst;
-    if (mode == 0) {
-        const T *ch = phi_in + ts + level_off(g, l + 1) + row;
-        src[0] = ch + (int64_t)(2 * c) * node_len;
-        src[1] = ch + (int64_t)(2 * c + 1) * node_len;
-        mi[0] = 0; mi[1] = 1;
-        nt = 2;
-        dst = phi_out + ts + level_off(g, l) + (int64_t)c * node_len + row;
-    } else if (mode == 1) {
-        const T *ph = phi_in + ts + level_off(g, l) + row;
-        src[0] = psi_in + ts + level_off(g, l - 1) + (int64_t)(c >> 1) * node_len + row;
-        if ((c & 1) == 0) {
-            mi[0] = 0;
-            mi[1] = 2; src[1] = ph + (int64_t)pmod(c - 2, nn) * node_len;
-            mi[2] = 3; src[2] = ph + (int64_t)pmod(c + 2, nn) * node_len;
-            mi[3] = 4; src[3] = ph + (int64_t)pmod(c + 3, nn) * node_len;
-        } else {
-            mi[0] = 1;
-            mi[1] = 5; src[1] = ph + (int64_t)pmod(c - 3, nn) * node_len;
-            mi[2] = 2; src[2] = ph + (int64_t)pmod(c - 2, nn) * node_len;
-            mi[3] = 3; src[3] = ph + (int64_t)pmod(c + 2, nn) * node_len;
-        }
-        nt = 4;
-        dst = psi_out + ts + level_off(g, l) + (int64_t)c * node_len + row;
-    } else {
-        src[0] = phi_in + ts + level_off(g, 2) + (int64_t)((c + 2) & 3) * node_len + row;
-        mi[0] = 0;
-        nt = 1;
-        dst = psi_out + ts + level_off(g, 2) + (int64_t)c * node_len + row;
-    }
-
-    T acc[PT];
-#pragma unroll
-    for (int k = 0; k < PT; ++k) acc[k] = T(0);
-#pragma unroll
-    for (int t = 0; t < 4; ++t) {
-        if (t < nt) {
-            T v[PT];
-#pragma unroll
-            for (int kk = 0; kk < PT; ++kk) v[kk] = kk < terms ? src[t][(int64_t)kk * rows] : T(0);
-            const T *A = As + mi[t] * PT * PT;
-#pragma unroll
-            for (int k = 0; k < PT; ++k) {
-                T a = acc[k];
-#pragma unroll
-                for (int kk = 0; kk < PT; ++kk) a += A[k * PT + kk] * v[kk];
-                acc[k] = a;
-            }
-        }
-    }
-#pragma unroll
-    for (int k = 0; k < PT; ++k)
-        if (k < terms) dst[(int64_t)k * rows] = acc[k];
-}
-
-// ---------------------------------------------------------------------------
 // Leaf evaluation: psec[sc][mu] = L2T(local expansion of the home box) +
 // sum over the 3b near sources of K_sc(nu - mu) x_sc[nu],
 // K_sc(d) = -cot(pi (d + sc/P) / M) / M  (src/fmmfft.c:751-954).
@@ -445,8 +328,9 @@ __global__ __launch_bounds__(256) void k_leaf_mma(Geom g, const cx<T> *__restric
     constexpr int ROWS = (BB + 3) * b;
     constexpr int XN = ROWS * G, XPER = (XN + 255) / 256;           // x window
     constexpr int KVN = G * 4 * b, KPER = (KVN + 255) / 256;        // kernel samples
-    // local expansions in LDS with the terms padded to TP = 4 KT (zeros past
-    // terms), so every index below is a compile-time shift or mask
+    // local expansions in LDS with the terms padded to TP = 4 KT (the padding
+    // slots hold copies of the last term and meet zero L2T rows), so every
+    // index below is a compile-time shift or mask
     constexpr int TP = 4 * KT;
     constexpr int NL = TP * 2 * G;                                   // one box
     constexpr int LSTR = NL + 2;                                     // +2: bank shift
@@ -497,8 +381,7 @@ __global__ __launch_bounds__(256) void k_leaf_mma(Geom g, const cx<T> *__restric
             if (KVN % 256 == 0 || e < KVN) {
                 const int gg = e / (4 * b), d = e % (4 * b);
                 const int sc = sc0 + gg;
-                const T v = kv[max(sc - 1, 0) * 4 * b + d];
-                kvr[u] = sc == 0 ? T(0) : v;
+                kvr[u] = kv[max(sc - 1, 0) * 4 * b + d];  // section 0 never reads it
             }
         }
 #pragma unroll
@@ -509,9 +392,10 @@ __global__ __launch_bounds__(256) void k_leaf_mma(Geom g, const cx<T> *__restric
                 const int k = rem / (2 * G), rr = rem % (2 * G);
                 const int sc = sc0 + (rr >> 1);
                 const int box = (i0 - 1 + bx9) & smask;
-                // clamped, unconditional load; padding terms and section 0 read as 0
-                const T v = psi[(box * terms + min(k, terms - 1)) * rowsS + max(2 * (sc - 1) + (rr & 1), 0)];
-                lr[u] = (k < terms && sc != 0) ? v : T(0);
+                // clamped, unconditional load (a lane-conditional load becomes
+                // a branch that waits for it): padding terms meet zero L2T
+                // rows, section 0 never reads its slot
+                lr[u] = psi[(box * terms + min(k, terms - 1)) * rowsS + max(2 * (sc - 1) + (rr & 1), 0)];
             }
         }
     };
@@ -669,6 +553,130 @@ __global__ __launch_bounds__(256) void k_leaf_mma(Geom g, const cx<T> *__restric
             }
         }
         lid = nlid;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Tree passes on the matrix cores, one launch per level:
+//   mode 0, upward, parent level l (src/fmmfft.c:476-515, mpp :174-184):
+//     phi_l[q] = shftfn . phi_{l+1}[2q] + shftfp . phi_{l+1}[2q+1]
+//   mode 1, downward, level l >= 3 (src/fmmfft.c:644-749):
+//     even c: shftlp . psi_{l-1}[c/2] + flip2p phi_l[c-2] + flip2n phi_l[c+2] + flip3n phi_l[c+3]
+//     odd  c: shftln . psi_{l-1}[c/2] + flip3p phi_l[c-3] + flip2p phi_l[c-2] + flip2n phi_l[c+2]
+//     node indices periodic (the reference's ring of ranks)
+//   mode 2, top flip (src/fmmfft.c:620-639): psi_2[q] = topflp . phi_2[q+2 mod 4]
+// Every contraction of a level is out[node][k][row] = sum_t sum_kk
+// A_t[k][kk] src_t[node][kk][row] with at most four sources t per node and
+// one operator per source that depends only on the node's parity, so for a
+// node and a block of 16 rows it is D(16 terms x 16 rows) = sum_t A_t B_t
+// with A_t the (zero-padded) operator, read in fragment order (one 256-byte
+// load per K step), and B_t read straight from the tree (lane l:
+// B[4s + l/16][l%16] = term 4s + l/16 of row 16 rb + l%16).  A wave takes RB
+// consecutive row blocks of one node and issues all their loads at once, so
+// a level costs about one memory round trip plus its HBM time; consecutive
+// waves take the rest of the node, then its neighbours, so the M2L sources
+// c +- 2, 3 are shared in one XCD's L2.  TP = terms padded to 16 or 32:
+// TP/4 K steps, TP/16 output term tiles.
+// ---------------------------------------------------------------------------
+template <typename T, int TP>
+__host__ __device__ constexpr int tree_mma_rb(int mode) {
+    // row blocks per wave: the B fragments of one wave stay within ~64 VGPRs
+    return mode == 2 ? 4 : (64 * 4 / ((mode == 0 ? 2 : 4) * (TP / 4) * (int)sizeof(T))) >= 4 ? 4
+                       : (64 * 4 / ((mode == 0 ? 2 : 4) * (TP / 4) * (int)sizeof(T))) >= 2 ? 2 : 1;
+}
+
+template <typename T, int TP, int MODE>
+__global__ __launch_bounds__(256) void k_tree_mma(Geom g, int l, const T *__restrict__ phi_in,
+                                                  T *__restrict__ phi_out, const T *__restrict__ psi_in,
+                                                  T *__restrict__ psi_out, int64_t tree_stride,
+                                                  const T *__restrict__ opfrag) {
+    using V4 = typename Mfma<T>::v4;
+    constexpr int KS = TP / 4, KO = TP / 16;
+    constexpr int NT = MODE == 0 ? 2 : MODE == 1 ? 4 : 1;  // sources per node
+    constexpr int RB = tree_mma_rb<T, TP>(MODE);
+    constexpr int FR = TP * TP;                             // one operator in fragment order
+    const int terms = g.terms, rows = 2 * g.S;
+    const int node_len = rows * terms;
+    const int nn = 1 << l, nrb = (rows + 15) >> 4, nw = (nrb + RB - 1) / RB;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int col = lane & 15, kr = lane >> 4;
+    const int wid = 4 * (int)xcd_remap(blockIdx.x, gridDim.x) + wave;
+    const int c = wid / nw, rb0 = (wid - c * nw) * RB;
+    if (c >= nn) return;
+    const int64_t ts = (int64_t)blockIdx.z * tree_stride;
+    const int odd = c & 1;
+
+    const T *mats[NT];
+    const T *src[NT];
+    T *dst;
+    if constexpr (MODE == 0) {
+        const T *o = opfrag + (size_t)(l - 2) * NOPS * FR;
+        mats[0] = o + OP_SHFTFN * FR;
+        mats[1] = o + OP_SHFTFP * FR;
+        const T *ch = phi_in + ts + level_off(g, l + 1);
+        src[0] = ch + (int64_t)(2 * c) * node_len;
+        src[1] = ch + (int64_t)(2 * c + 1) * node_len;
+        dst = phi_out + ts + level_off(g, l) + (int64_t)c * node_len;
+    } else if constexpr (MODE == 1) {
+        const T *o = opfrag + (size_t)(l - 3) * NOPS * FR;
+        mats[0] = o + (odd ? OP_SHFTLN : OP_SHFTLP) * FR;
+        mats[1] = o + (odd ? OP_FLIP3P : OP_FLIP2P) * FR;
+        mats[2] = o + (odd ? OP_FLIP2P : OP_FLIP2N) * FR;
+        mats[3] = o + (odd ? OP_FLIP2N : OP_FLIP3N) * FR;
+        const T *ph = phi_in + ts + level_off(g, l);
+        const int m = nn - 1;
+        src[0] = psi_in + ts + level_off(g, l - 1) + (int64_t)(c >> 1) * node_len;
+        src[1] = ph + (int64_t)((c + (odd ? -3 : -2)) & m) * node_len;
+        src[2] = ph + (int64_t)((c + (odd ? -2 : 2)) & m) * node_len;
+        src[3] = ph + (int64_t)((c + (odd ? 2 : 3)) & m) * node_len;
+        dst = psi_out + ts + level_off(g, l) + (int64_t)c * node_len;
+    } else {
+        mats[0] = opfrag + (size_t)(g.n - 2) * NOPS * FR;  // topflp
+        src[0] = phi_in + ts + level_off(g, 2) + (int64_t)((c + 2) & 3) * node_len;
+        dst = psi_out + ts + level_off(g, 2) + (int64_t)c * node_len;
+    }
+    // all loads of the wave at once, unconditional (a load under a lane
+    // condition becomes a branch that waits for it): operator fragments,
+    // then the RB row blocks' source fragments at clamped indices.  No
+    // zeroing is needed: rows past the end only feed output columns that are
+    // not stored, and terms past the end meet zero operator columns.
+    T a[NT][KO][KS], bv[RB][NT][KS];
+#pragma unroll
+    for (int t = 0; t < NT; ++t)
+#pragma unroll
+        for (int s = 0; s < KS; ++s)
+#pragma unroll
+            for (int ko = 0; ko < KO; ++ko) a[t][ko][s] = mats[t][(s * KO + ko) * 64 + lane];
+#pragma unroll
+    for (int j = 0; j < RB; ++j) {
+        const int rc = min(16 * (rb0 + j) + col, rows - 1);
+#pragma unroll
+        for (int t = 0; t < NT; ++t)
+#pragma unroll
+            for (int s = 0; s < KS; ++s) bv[j][t][s] = src[t][min(4 * s + kr, terms - 1) * rows + rc];
+    }
+#pragma unroll
+    for (int j = 0; j < RB; ++j) {
+        const int row = 16 * (rb0 + j) + col;
+        if (16 * (rb0 + j) >= rows) break;  // wave-uniform
+        V4 acc[KO];
+#pragma unroll
+        for (int ko = 0; ko < KO; ++ko) acc[ko] = V4{0, 0, 0, 0};
+#pragma unroll
+        for (int t = 0; t < NT; ++t)
+#pragma unroll
+            for (int s = 0; s < KS; ++s)
+#pragma unroll
+                for (int ko = 0; ko < KO; ++ko) acc[ko] = Mfma<T>::mma(a[t][ko][s], bv[j][t][s], acc[ko]);
+        if (row < rows) {
+#pragma unroll
+            for (int ko = 0; ko < KO; ++ko)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int k = 16 * ko + Mfma<T>::row(kr, r);
+                    if (k < terms) dst[k * rows + row] = acc[ko][r];
+                }
+        }
     }
 }
 

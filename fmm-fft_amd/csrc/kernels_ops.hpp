@@ -231,6 +231,25 @@ __global__ __launch_bounds__(256) void k_build_evfrag(Geom g, const double *eval
     evfrag[e] = k < g.terms ? evalm[(sc1 * g.b + j) * g.terms + k] : 0.0;
 }
 
+// Tree operators in matrix-core fragment order for k_tree_mma: op j (slot
+// L = j / NOPS, operator j % NOPS; j = (n-2) NOPS is topflp), K step s,
+// output tile ko, lane l: A[k = 16 ko + l%16][kk = 4 s + l/16], zero past
+// terms.  tp = terms padded to 16 or 32; one operator is tp*tp entries.
+__global__ __launch_bounds__(256) void k_build_opfrag(Geom g, const double *ops, const double *topflp,
+                                                      double *opfrag, int tp) {
+    const int nop = (g.n - 2) * NOPS + 1;
+    const int per = tp * tp, ko_n = tp / 16;
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (int64_t)nop * per) return;
+    const int j = (int)(e / per), r = (int)(e % per);
+    const int lane = r & 63, f = r >> 6;  // f = s * ko_n + ko
+    const int s = f / ko_n, ko = f % ko_n;
+    const int k = 16 * ko + (lane & 15), kk = 4 * s + (lane >> 4);
+    const int T = g.terms;
+    const double *op = j < nop - 1 ? ops + (size_t)j * T * T : topflp;
+    opfrag[e] = (k < T && kk < T) ? op[k * T + kk] : 0.0;
+}
+
 __global__ __launch_bounds__(256) void k_to_f32(const double *src, float *dst, int64_t n) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) dst[i] = (float)src[i];
