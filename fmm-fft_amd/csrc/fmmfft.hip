@@ -146,7 +146,7 @@ size_t leaf_lds(const Geom &g, size_t esize, int G, int bb) {
 }
 
 size_t leaf_mma_lds(const Geom &g, size_t esize, int G) {
-    return ((size_t)11 * g.b * G + 11) * 2 * esize + (size_t)G * 4 * g.b * esize +
+    return ((size_t)11 * g.b * G + 19) * 2 * esize + (size_t)G * 4 * g.b * esize +
            (size_t)9 * (4 * leaf_kt(g.terms) * 2 * G + 2) * esize;
 }
 // (the persistent leaf kernel's LDS is at most 11*128*4*16 + ... < 100 KB)

@@ -343,8 +343,14 @@ __global__ __launch_bounds__(256) void k_leaf_mma(Geom g, const cx<T> *__restric
     // offsets inside one signal fit 32 bits (N <= 2^26); P, M and s are
     // powers of two
     const int lp = __builtin_ctz(P), mmask = (int)M - 1, smask = g.s - 1;
-    cx<T> *xw = reinterpret_cast<cx<T> *>(smem);           // [ROWS][G] + 1 pad per box
-    T *kvs = reinterpret_cast<T *>(xw + XN + BB + 3);      // [G][4b]
+    // window rows of box q start at q b G + boxpad(q): for 8 consecutive
+    // boxes the pads cover 8 distinct bank pairs apart from the row stride's
+    // bank, so the B loads (lanes over 8 boxes x re/im x 4 rows) are free of
+    // LDS bank conflicts
+    auto boxpad = [](int q) { return (q & 3) + 8 * (q >> 2); };
+    constexpr int XPAD = ((BB + 2) & 3) + 8 * ((BB + 2) >> 2) + 1;
+    cx<T> *xw = reinterpret_cast<cx<T> *>(smem);           // [ROWS][G] + pads
+    T *kvs = reinterpret_cast<T *>(xw + XN + XPAD);        // [G][4b]
     T *lsx = kvs + KVN;                                    // [BB+1][TP][G][2] (+2 per box)
     const int sig = blockIdx.z;
     x += (int64_t)sig * x_stride;
@@ -406,7 +412,7 @@ __global__ __launch_bounds__(256) void k_leaf_mma(Geom g, const cx<T> *__restric
             if (XN % 256 == 0 || e < XN) {
                 cx<T> v = xr[u];
                 if (conj) v.im = -v.im;
-                xw[e + (e / G) / b] = v;  // one pad slot per box: box offsets land on distinct banks
+                xw[e + boxpad((e / G) / b)] = v;
             }
         }
 #pragma unroll
@@ -441,7 +447,7 @@ __global__ __launch_bounds__(256) void k_leaf_mma(Geom g, const cx<T> *__restric
             const int sc = sc0 + gg;
             if (sc == 0) {
                 cx<T> *dst = psec + (int64_t)i0 * b;
-                for (int t = lane; t < BB * b; t += 64) dst[t] = xw[(2 * b + t) * G + (2 * b + t) / b];
+                for (int t = lane; t < BB * b; t += 64) dst[t] = xw[(2 * b + t) * G + boxpad((2 * b + t) / b)];
                 continue;
             }
             const T *kr_ = kvs + gg * 4 * b;
@@ -468,7 +474,7 @@ __global__ __launch_bounds__(256) void k_leaf_mma(Geom g, const cx<T> *__restric
                 for (int t = 0; t < 4; ++t) areg[rt][t] = kr_[4 * t + kr + b - 16 * rt - col];
             // window rows of group gi: (bx+1) b + 16 gi + 4 t + kr (+ box pad)
             auto xrow = [&](int gi) {
-                return xwc + (((size_t)(bx + 1) * b + 16 * gi) * G + gg + (bx + 1 + gi / (b / 16))) * 2 + c;
+                return xwc + (((size_t)(bx + 1) * b + 16 * gi) * G + gg + boxpad(bx + 1 + gi / (b / 16))) * 2 + c;
             };
             // B loads run one group ahead of the MFMAs that use them
             T bvc[4];
@@ -517,8 +523,8 @@ __global__ __launch_bounds__(256) void k_leaf_mma(Geom g, const cx<T> *__restric
                 // re-home target 0 of box bx to box bx-1: lanes col, col+16,
                 // col+32, col+48 each take a quarter of the correction terms
                 T corr = 0;
-                const T *xm2 = xwc + ((size_t)bx * b * G + gg + bx) * 2 + c;            // box i-2
-                const T *xp1 = xwc + ((size_t)(bx + 3) * b * G + gg + bx + 3) * 2 + c;  // box i+1
+                const T *xm2 = xwc + ((size_t)bx * b * G + gg + boxpad(bx)) * 2 + c;              // box i-2
+                const T *xp1 = xwc + ((size_t)(bx + 3) * b * G + gg + boxpad(bx + 3)) * 2 + c;    // box i+1
                 for (int t = kr; t < b; t += 4)
                     corr += kr_[t] * xm2[(size_t)t * G * 2] - kr_[3 * b + t] * xp1[(size_t)t * G * 2];
                 const T *lm = lsx + bx * LSTR + 2 * gg + c;  // box i-1
