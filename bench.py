@@ -227,8 +227,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-log2n", type=int, default=23)
     ap.add_argument("--no-serve", action="store_true", help="skip the mixed request stream")
-    ap.add_argument("--serve-requests", type=int, default=3000,
-                    help="requests per serving phase (the full serving config uses 20000)")
+    ap.add_argument("--serve-requests", type=int, default=20000,
+                    help="requests per serving phase (the serving config's 20,000)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="after timing, copy the complex64 input twice with torch (N*8 bytes read and written "
                          "per copy): the reference dispatch for scaling HBM byte counters")

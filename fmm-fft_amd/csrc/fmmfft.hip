@@ -154,9 +154,11 @@ size_t leaf_mma_lds(const Geom &g, size_t esize, int G) {
 Shapes shapes(const Geom &g, size_t esize) {
     Shapes sh{};
     const size_t lds_cap = 64 * 1024;
-    const int per = 256 / g.P;  // boxes in flight per S2M workgroup
-    sh.s2m_bpb = per * std::max(1, g.s / (per * 2048));
-    sh.s2m_nblk = (g.s + sh.s2m_bpb - 1) / sh.s2m_bpb;
+    // S2M: tiles of 64 KB of x (bx boxes of all sections), at most 512
+    // persistent workgroups (one partial section sum each)
+    const int s2m_bx = std::min<int>((int)(65536 / (2 * esize) / ((size_t)g.b * g.P)), g.s);
+    sh.s2m_bpb = s2m_bx;
+    sh.s2m_nblk = std::min(g.s / s2m_bx, 512);
     sh.s2m_lds = (size_t)g.b * padded_terms(g.terms) * esize;
 
     int G = std::min(g.P, esize == 4 ? 16 : 8);
@@ -366,8 +368,8 @@ template <typename T, int PT, typename Mark>
 void s2m_tree_pt(const Geom &g, const Shapes &sh, unsigned nz, hipStream_t st, const cx<T> *x, T *phi, T *psi,
                  int64_t wstride_T, c64 *sums_part, int64_t wstride_c64, c64 *sumsec, const TabPtrs<T> &tb,
                  int conj, Mark &mark) {
-    hipLaunchKernelGGL((k_s2m<T, PT>), dim3(sh.s2m_nblk, 1, nz), dim3(256), sh.s2m_lds, st, g, x, g.N, phi,
-                       wstride_T, sums_part, wstride_c64, tb.initch, sh.s2m_bpb, conj);
+    hipLaunchKernelGGL((k_s2m<T, PT>), dim3(sh.s2m_nblk, 1, nz), dim3(kS2mThreads), sh.s2m_lds, st, g, x, g.N,
+                       phi, wstride_T, sums_part, wstride_c64, tb.initch, conj);
     hipLaunchKernelGGL(k_sums_reduce, dim3(g.P, 1, nz), dim3(256), 0, st, g, sums_part, sh.s2m_nblk, wstride_c64,
                        sumsec, wstride_c64);
     mark(1);

@@ -163,6 +163,21 @@ __device__ inline void stockham(cx<T> *a, int ls, const cx<T> *__restrict__ twL)
 template <int L>
 constexpr int row_stride() { return L + L / 16 + 1; }
 
+// Row stride (in elements) of a tile of C rows of length L holding elements
+// of `esize` bytes.  Inside a row one pad slot per 16 elements (lpad) breaks
+// the radix-16 stride; across rows the stride is chosen so that a transposed
+// access (consecutive lanes step through C' = min(C, Q) rows, then along
+// the row) hits Q distinct bank groups per LDS cycle, Q = 128 / esize lanes:
+// stride = Q / C' (mod Q).
+template <int L, int C, int ESIZE>
+constexpr int tile_stride() {
+    constexpr int Q = 128 / ESIZE;
+    constexpr int CQ = C < Q ? C : Q;
+    constexpr int base = L + L / 16;
+    constexpr int want = Q / CQ;
+    return base + (((want - base) % Q) + Q) % Q;
+}
+
 // Sub-FFTs per workgroup: 8192 (complex64) or 4096 (complex128) elements,
 // at most 32; about 70 KB of LDS.
 template <typename T, int L>
@@ -197,7 +212,7 @@ __global__ __launch_bounds__(kFftThreads) void k_fft(Geom g, FftPass fp, const c
                                              cx<T> *__restrict__ out, int64_t in_stride, int64_t out_stride,
                                              const cx<T> *__restrict__ twL, const c64 *__restrict__ tlo,
                                              const c64 *__restrict__ thi) {
-    constexpr int ls = row_stride<L>();
+    constexpr int ls = tile_stride<L, C, (int)sizeof(cx<T>)>();
     constexpr int n = C * L;
     constexpr int NT = kFftThreads;
     constexpr int PER = n / NT;

@@ -16,79 +16,131 @@
 #include "common.hpp"
 #include "kernels_fft.hpp"
 
+#include <type_traits>
+
 namespace fmmfft {
 
 // ---------------------------------------------------------------------------
-// S2M + per-block section partial sums.
+// S2M + per-workgroup section partial sums.
 // The reference's initmm (src/fmmfft.c:199-211, called at :465):
 //   phi_leaf[box][sc][k] = sum_j x_sc[box*b + j] * initch[j][k]
 // and its per-rank section sums (src/fmmfft.c:441-450), accumulated in FP64.
-// A lane owns one (box, section): it streams the section's b samples straight
-// from HBM (lanes of a wave read consecutive sections of one row: coalesced),
-// keeps 2*terms accumulators in registers and reads initch from LDS
-// (broadcast).  The workgroup covers 256/P boxes at a time and loops over
-// `bpb` boxes in total.
+// A tile is BX consecutive boxes of all P sections, i.e. one contiguous
+// 64 KB range of x (BX = 64 KB / (b P esize), at least 1 by the plan's
+// leaf * P limit).  A persistent workgroup streams its tiles with 16-byte
+// loads into registers, one tile ahead, commits a tile to LDS and then
+// computes work items (box, section, group of 4 terms): b samples read from
+// LDS (lanes = consecutive sections: no bank conflicts) against initch rows
+// (broadcast), 8 accumulators per item.  Items with term group 0 also add
+// the samples into the FP64 section sums.
 // ---------------------------------------------------------------------------
+constexpr int kS2mThreads = 512;
+
 template <typename T, int PT>
-__global__ __launch_bounds__(256) void k_s2m(Geom g, const cx<T> *__restrict__ x, int64_t x_stride,
-                                             T *__restrict__ phi, int64_t tree_stride,
-                                             c64 *__restrict__ sums_part, int64_t sums_stride,
-                                             const T *__restrict__ initch, int bpb, int conj) {
+__global__ __launch_bounds__(kS2mThreads) void k_s2m(Geom g, const cx<T> *__restrict__ x, int64_t x_stride,
+                                                     T *__restrict__ phi, int64_t tree_stride,
+                                                     c64 *__restrict__ sums_part, int64_t sums_stride,
+                                                     const T *__restrict__ initch, int conj) {
+    using V = typename std::conditional<sizeof(T) == 4, float4, double2>::type;  // 16 bytes
+    constexpr int NT = kS2mThreads;
+    constexpr int TILE = 65536 / (int)sizeof(cx<T>);   // complex elements per tile (64 KB)
+    constexpr int NV = 65536 / 16, VPER = NV / NT;      // 16-byte vectors
+    constexpr int KG = PT / 4;                          // term groups of 4
+    __shared__ __align__(16) cx<T> xs[TILE];
+    __shared__ double red[2][NT];
     extern __shared__ __align__(16) unsigned char smem[];
+    T *ich = reinterpret_cast<T *>(smem);  // [b][PT], zero padded
     const int P = g.P, b = g.b, terms = g.terms;
     const int rows = 2 * g.S;
-    T *ich = reinterpret_cast<T *>(smem);  // [b][PT], zero padded
-    __shared__ double red[2][256];
+    const int bx_n = min(TILE / (b * P), g.s);          // boxes per tile
+    const int tile_elems = bx_n * b * P, tile_vecs = tile_elems * (int)sizeof(cx<T>) / 16;
+    const int ntiles = g.s / bx_n;
     const int sig = blockIdx.z;
     x += (int64_t)sig * x_stride;
     phi += (int64_t)sig * tree_stride + level_off(g, g.n);
     sums_part += (int64_t)sig * sums_stride + (int64_t)blockIdx.x * P;
-    for (int i = threadIdx.x; i < b * PT; i += blockDim.x) {
+    for (int i = threadIdx.x; i < b * PT; i += NT) {
         const int j = i / PT, k = i - j * PT;
         ich[i] = k < terms ? initch[j * terms + k] : T(0);
     }
-    __syncthreads();
 
-    const int per = blockDim.x / P;  // boxes in flight
-    const int slot = threadIdx.x / P, sc = threadIdx.x - slot * P;
+    V v[VPER];
+    auto prefetch = [&](int t) __attribute__((always_inline)) {
+        const V *src = reinterpret_cast<const V *>(x + (int64_t)t * tile_elems);
+#pragma unroll
+        for (int u = 0; u < VPER; ++u) {
+            const int e = threadIdx.x + u * NT;
+            v[u] = src[min(e, tile_vecs - 1)];
+        }
+    };
     double sre = 0.0, sim = 0.0;
-    const int box_end = min(g.s, (blockIdx.x + 1) * bpb);
-    for (int box = blockIdx.x * bpb + slot; slot < per && box < box_end; box += per) {
-        const cx<T> *src = x + (int64_t)box * b * P + sc;
-        T are[PT], aim[PT];
+    const int n_items = bx_n * P * KG;
+    int t = blockIdx.x;
+    if (t < ntiles) prefetch(t);
+    while (t < ntiles) {
+        __syncthreads();  // the previous tile is no longer read (and ich is staged)
+        V *xv = reinterpret_cast<V *>(xs);
 #pragma unroll
-        for (int k = 0; k < PT; ++k) are[k] = aim[k] = T(0);
+        for (int u = 0; u < VPER; ++u) {
+            const int e = threadIdx.x + u * NT;
+            V w = v[u];
+            if (conj) {
+                if constexpr (sizeof(T) == 4) {
+                    w.y = -w.y;
+                    w.w = -w.w;
+                } else {
+                    w.y = -w.y;
+                }
+            }
+            if (e < tile_vecs) xv[e] = w;
+        }
+        __syncthreads();
+        const int tn = t + gridDim.x;
+        if (tn < ntiles) prefetch(tn);
+        for (int it = threadIdx.x; it < n_items; it += NT) {
+            const int sc = it % P, kg = (it / P) % KG, bx = it / (P * KG);
+            const cx<T> *col = xs + bx * b * P + sc;
+            const T *ir0 = ich + 4 * kg;
+            T are[4] = {0, 0, 0, 0}, aim[4] = {0, 0, 0, 0};
+            double pre = 0.0, pim = 0.0;
 #pragma unroll 8
-        for (int j = 0; j < b; ++j) {
-            cx<T> v = src[(int64_t)j * P];
-            if (conj) v.im = -v.im;
-            sre += (double)v.re;
-            sim += (double)v.im;
-            const T *ir = ich + j * PT;
+            for (int j = 0; j < b; ++j) {
+                const cx<T> xv1 = col[j * P];
+                const T *ir = ir0 + j * PT;
 #pragma unroll
-            for (int k = 0; k < PT; ++k) {
-                are[k] += v.re * ir[k];
-                aim[k] += v.im * ir[k];
+                for (int k = 0; k < 4; ++k) {
+                    are[k] += xv1.re * ir[k];
+                    aim[k] += xv1.im * ir[k];
+                }
+                pre += (double)xv1.re;
+                pim += (double)xv1.im;
+            }
+            if (kg == 0) {
+                sre += pre;
+                sim += pim;
+            }
+            if (sc > 0) {
+                T *dst = phi + (int64_t)(t * bx_n + bx) * terms * rows + 2 * (sc - 1);
+#pragma unroll
+                for (int k = 0; k < 4; ++k)
+                    if (4 * kg + k < terms) {
+                        dst[(int64_t)(4 * kg + k) * rows] = are[k];
+                        dst[(int64_t)(4 * kg + k) * rows + 1] = aim[k];
+                    }
             }
         }
-        if (sc > 0) {
-            T *dst = phi + (int64_t)box * terms * rows + 2 * (sc - 1);
-#pragma unroll
-            for (int k = 0; k < PT; ++k)
-                if (k < terms) {
-                    dst[(int64_t)k * rows] = are[k];
-                    dst[(int64_t)k * rows + 1] = aim[k];
-                }
-        }
+        t = tn;
     }
+    // per-section partial sums of this workgroup: threads with equal it % P
+    // (and term group 0) hold them; fixed-order reduction
     red[0][threadIdx.x] = sre;
     red[1][threadIdx.x] = sim;
     __syncthreads();
-    if (threadIdx.x < P) {
+    if ((int)threadIdx.x < P) {
         double re = 0.0, im = 0.0;
-        for (int s2 = 0; s2 < per; ++s2) {
-            re += red[0][s2 * P + threadIdx.x];
-            im += red[1][s2 * P + threadIdx.x];
+        for (int q = threadIdx.x; q < NT; q += P) {
+            re += red[0][q];
+            im += red[1][q];
         }
         sums_part[threadIdx.x] = c64{re, im};
     }
