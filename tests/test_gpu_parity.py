@@ -105,6 +105,8 @@ SMALL = [
     (1 << 16, 16, 12, 32),  # M = 4096
     (1 << 17, 8, 16, 64),   # two-pass local FFT
     (1 << 18, 64, 16, 32),
+    (1 << 16, 16, 16, 16),  # leaf 16: S2M tiles of many boxes
+    (1 << 15, 32, 20, 16),
 ]
 
 
