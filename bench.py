@@ -202,6 +202,26 @@ def cpu_baseline(sample_log2n):
     }
 
 
+def cpu_fft_baseline(log2n):
+    """A plain FP64 FFT on the host (numpy's pocketfft, one thread; FFTW is
+    not installed), the role of the reference's fftw_serial.c: 1 warm-up and
+    3 timed runs on N = 2^log2n complex128, median."""
+    import numpy as np
+
+    n = 1 << log2n
+    rng = np.random.default_rng(1)
+    x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    np.fft.fft(x)
+    ts = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        np.fft.fft(x)
+        ts.append(time.perf_counter() - t0)
+    t = sorted(ts)[1]
+    return {"value": round(5 * n * math.log2(n) / t / 1e9, 3), "unit": "GFLOP/s", "cores": 1,
+            "sample": f"numpy.fft.fft (pocketfft), N=2^{log2n} complex128, median of 3, {t:.3f} s per transform"}
+
+
 def measured_traffic(stage):
     """HBM bytes per launch of one stage's kernel at this workload, as
     measured by the rocprofv3 FETCH_SIZE and WRITE_SIZE passes summarised in
@@ -329,6 +349,7 @@ def main():
         }
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(args.cpu_sample_log2n)
+        line["cpu_fft_baseline"] = cpu_fft_baseline(24)
     print(json.dumps(line), flush=True)
 
 

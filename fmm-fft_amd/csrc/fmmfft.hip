@@ -686,6 +686,8 @@ int fmmfft_execute_ws(fmmfft_plan plan, int64_t count, const void *d_in, void *d
     if (!plan || !d_in || !d_out || !d_work || count <= 0 || count > 65535) return FMMFFT_EINVAL;
     if (dir != FMMFFT_FORWARD && dir != FMMFFT_INVERSE) return FMMFFT_EINVAL;
     if (work_bytes < plan->ws.stride * (size_t)count || ((uintptr_t)d_work % kAlign) != 0) return FMMFFT_EINVAL;
+    // the S2M kernel streams the input with 16-byte loads
+    if (((uintptr_t)d_in % 16) != 0 || ((uintptr_t)d_out % (2 * plan->prm.esize)) != 0) return FMMFFT_EINVAL;
     hipStream_t st = static_cast<hipStream_t>(stream);
     if (plan->prm.dtype == FMMFFT_C64) return run<float>(plan, count, d_in, d_out, dir, (char *)d_work, st);
     return run<double>(plan, count, d_in, d_out, dir, (char *)d_work, st);

@@ -82,8 +82,9 @@ int fmmfft_plan_destroy(fmmfft_plan plan);
 
 /* One transform (replaces mft, src/fmmfft.c:980).  d_in, d_out: device
  * pointers to N complex values (complex64 or complex128 per the plan),
- * natural order; d_in == d_out is allowed.  stream: a hipStream_t (NULL =
- * default stream). */
+ * natural order; d_in == d_out is allowed.  d_in must be 16-byte aligned
+ * (FMMFFT_EINVAL otherwise; device allocations are), d_out aligned to its
+ * element.  stream: a hipStream_t (NULL = default stream). */
 int fmmfft_execute(fmmfft_plan plan, const void *d_in, void *d_out, int32_t dir,
                    void *stream);
 

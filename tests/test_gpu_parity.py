@@ -205,6 +205,12 @@ def test_invalid_arguments_raise(fmm, torch):
         plan.execute(x)
     with pytest.raises(fmm.FmmFftError):
         fmm.Plan(1 << 14, 3, 16, 32, fmm.C128)
+    # an input that is not 16-byte aligned (complex64 view at an odd element)
+    p64 = fmm.Plan(1 << 14, 4, 16, 32, fmm.C64)
+    buf = torch.zeros((1 << 14) + 1, dtype=torch.complex64, device="cuda")
+    with pytest.raises(fmm.FmmFftError) as e:
+        p64.execute(buf[1:], torch.empty(1 << 14, dtype=torch.complex64, device="cuda"))
+    assert e.value.status == fmm.EINVAL
 
 
 # ---------------------------------------------------------------------------
