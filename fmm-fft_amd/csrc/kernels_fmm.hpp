@@ -638,7 +638,9 @@ __global__ __launch_bounds__(256) void k_leaf_mma(Geom g, const cx<T> *__restric
 // ---------------------------------------------------------------------------
 template <typename T, int TP>
 __host__ __device__ constexpr int tree_mma_rb(int mode) {
-    // row blocks per wave: the B fragments of one wave stay within ~64 VGPRs
+    // row blocks per wave: the B fragments of one wave stay within ~64 VGPRs;
+    // the FP32 downward pass measures best at 2 (more waves in flight)
+    if (mode == 1 && sizeof(T) == 4) return 2;
     return mode == 2 ? 4 : (64 * 4 / ((mode == 0 ? 2 : 4) * (TP / 4) * (int)sizeof(T))) >= 4 ? 4
                        : (64 * 4 / ((mode == 0 ? 2 : 4) * (TP / 4) * (int)sizeof(T))) >= 2 ? 2 : 1;
 }
