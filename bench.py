@@ -202,6 +202,33 @@ def cpu_baseline(sample_log2n):
     }
 
 
+def other_configs(torch, fmm):
+    """Throughput of the other GPU workloads: the plain FP64 drop-in (one
+    N=2^24 complex128 transform, P=32, p=20, leaf 64, N(0,1) input) and the
+    batched precision form (256 signals of N=2^20 complex64, P=16, p=12,
+    leaf 32, in one launch sequence), 5 timed steps after 2 warm-ups each."""
+    res = {}
+    g = torch.Generator(device="cuda").manual_seed(5)
+    n = 1 << 24
+    plan = fmm.Plan(n, 32, 20, 64, fmm.C128)
+    x = torch.randn(n, dtype=torch.complex128, device="cuda", generator=g)
+    out = torch.empty_like(x)
+    ms, _, _ = time_plan(torch, plan, x, out, 5, 2, False)
+    res["fp64_dropin_n2e24"] = {"gflops": round(5 * n * math.log2(n) / (ms * 1e-3) / 1e9, 1),
+                                "ms_per_transform": round(ms, 4)}
+    del plan, x, out
+    n, count = 1 << 20, 256
+    plan = fmm.Plan(n, 16, 12, 32, fmm.C64, max_batch=count)
+    x = torch.randn(count, n, dtype=torch.complex64, device="cuda", generator=g)
+    out = torch.empty_like(x)
+    ms, _, _ = time_plan(torch, plan, x, out, 5, 2, False)
+    res["batched_256x2e20_c64"] = {"gflops": round(count * 5 * n * math.log2(n) / (ms * 1e-3) / 1e9, 1),
+                                   "ms_per_batch": round(ms, 4)}
+    del plan, x, out
+    torch.cuda.empty_cache()
+    return res
+
+
 def cpu_fft_baseline(log2n):
     """A plain FP64 FFT on the host (numpy's pocketfft, one thread; FFTW is
     not installed), the role of the reference's fftw_serial.c: 1 warm-up and
@@ -249,6 +276,8 @@ def main():
     ap.add_argument("--no-serve", action="store_true", help="skip the mixed request stream")
     ap.add_argument("--serve-requests", type=int, default=20000,
                     help="requests per serving phase (the serving config's 20,000)")
+    ap.add_argument("--no-other-configs", action="store_true",
+                    help="skip the FP64 drop-in (N=2^24, P=32, p=20) and batched (256 x 2^20) timings")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="after timing, copy the complex64 input twice with torch (N*8 bytes read and written "
                          "per copy): the reference dispatch for scaling HBM byte counters")
@@ -291,6 +320,9 @@ def main():
             "rel_l2_fp32_vs_fp64": rel,
         })
         del plan64, xin64, out64, d
+
+    if not args.no_other_configs:
+        extra["other_configs"] = other_configs(torch, fmm)
 
     per_stage = {k: v / stages[1] for k, v in stages[0].items()}
     dom = max(per_stage, key=per_stage.get)
