@@ -145,9 +145,18 @@ size_t leaf_lds(const Geom &g, size_t esize, int G, int bb) {
            (size_t)(bb + 1) * G * 2 * g.terms * esize;
 }
 
+// Column tiles (of 8 boxes) per leaf tile: leaf 32 and 16 take 2 and 4 so
+// that a tile holds as many MFMAs as one of leaf 64 (at most s / 8).
+int leaf_ct(const Geom &g) {
+    const int ct = g.b >= 64 ? 1 : g.b == 32 ? 2 : 4;
+    return std::min(ct, g.s / 8);
+}
+
 size_t leaf_mma_lds(const Geom &g, size_t esize, int G) {
-    return ((size_t)11 * g.b * G + 19) * 2 * esize + (size_t)G * 4 * g.b * esize +
-           (size_t)9 * (4 * leaf_kt(g.terms) * 2 * G + 2) * esize;
+    const int bb = 8 * leaf_ct(g);
+    const int q = bb + 2, xpad = (q & 3) + 8 * (q >> 2) + 1;
+    return ((size_t)(bb + 3) * g.b * G + xpad) * 2 * esize + (size_t)G * 4 * g.b * esize +
+           (size_t)(bb + 1) * (4 * leaf_kt(g.terms) * 2 * G + 2) * esize;
 }
 // (the persistent leaf kernel's LDS is at most 11*128*4*16 + ... < 100 KB)
 
@@ -301,10 +310,11 @@ void launch_fft(int64_t L, const Geom &g, const FftPass &fp, unsigned nz, hipStr
 template <typename T>
 void launch_combine(const Geom &g, unsigned nz, hipStream_t st, const cx<T> *psec, int64_t ps, const c64 *sumsec,
                     int64_t ss, const c64 *gfac, const c64 *wP, cx<T> *y, int64_t ys) {
-    const int64_t ntiles = (g.M + 31) / 32;
 #define COMBINE(PP)                                                                                          \
     hipLaunchKernelGGL((k_combine<T, PP>),                                                                   \
-                       dim3(resident_grid((const void *)k_combine<T, PP>, kCombineThreads, 0, ntiles), 1, nz), \
+                       dim3(resident_grid((const void *)k_combine<T, PP>, kCombineThreads, 0,                  \
+                                          (g.M + combine_rows<PP>() - 1) / combine_rows<PP>()),                \
+                            1, nz),                                                                          \
                        dim3(kCombineThreads), 0, st, g, psec, ps, sumsec, ss, gfac, wP, y, ys)
     switch (g.P) {
         case 2: COMBINE(2); break;
@@ -317,10 +327,10 @@ void launch_combine(const Geom &g, unsigned nz, hipStream_t st, const cx<T> *pse
 #undef COMBINE
 }
 
-template <typename T, int NRT, int G, int KT>
+template <typename T, int NRT, int G, int KT, int CT>
 void leaf_mma_inst(const Geom &g, const Shapes &sh, unsigned nz, hipStream_t st, const cx<T> *x, const T *psi,
                    int64_t wstride_T, cx<T> *psec, int64_t wstride_cx, const TabPtrs<T> &tb, int conj) {
-    auto kern = k_leaf_mma<T, NRT, G, KT>;
+    auto kern = k_leaf_mma<T, NRT, G, KT, CT>;
     // persistent grid: as many workgroups as can be resident at once
     int dev = 0, cus = 0, per_cu = 0;
     (void)hipGetDevice(&dev);
@@ -329,7 +339,7 @@ void leaf_mma_inst(const Geom &g, const Shapes &sh, unsigned nz, hipStream_t st,
         (void)hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh.leaf_lds);
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 256, sh.leaf_lds) != hipSuccess || per_cu < 1)
         per_cu = 1;
-    const int64_t ntiles = (int64_t)(g.P / G) * (g.s / 8);
+    const int64_t ntiles = (int64_t)(g.P / G) * (g.s / (8 * CT));
     const unsigned nwg = (unsigned)std::min<int64_t>(ntiles, (int64_t)std::max(cus, 8) * per_cu);
     hipLaunchKernelGGL(kern, dim3(nwg, 1, nz), dim3(256), sh.leaf_lds, st, g, x, g.N, psi, wstride_T, psec,
                        wstride_cx, tb.evfrag, tb.evalm, tb.evalmh, tb.kv, conj);
@@ -338,10 +348,23 @@ void leaf_mma_inst(const Geom &g, const Shapes &sh, unsigned nz, hipStream_t st,
 template <typename T, int NRT, int G>
 void leaf_mma_kt(const Geom &g, const Shapes &sh, unsigned nz, hipStream_t st, const cx<T> *x, const T *psi,
                  int64_t wstride_T, cx<T> *psec, int64_t wstride_cx, const TabPtrs<T> &tb, int conj) {
-    if (leaf_kt(g.terms) == 4)
-        leaf_mma_inst<T, NRT, G, 4>(g, sh, nz, st, x, psi, wstride_T, psec, wstride_cx, tb, conj);
-    else
-        leaf_mma_inst<T, NRT, G, 8>(g, sh, nz, st, x, psi, wstride_T, psec, wstride_cx, tb, conj);
+    // column tiles: NRT 1 -> up to 4, NRT 2 -> up to 2 (leaf_ct); s >= 8
+    const int ct = leaf_ct(g);
+    if (leaf_kt(g.terms) == 4) {
+        if (NRT == 1 && ct == 4)
+            leaf_mma_inst<T, NRT, G, 4, (NRT == 1 ? 4 : 1)>(g, sh, nz, st, x, psi, wstride_T, psec, wstride_cx, tb, conj);
+        else if (NRT <= 2 && ct >= 2)
+            leaf_mma_inst<T, NRT, G, 4, (NRT <= 2 ? 2 : 1)>(g, sh, nz, st, x, psi, wstride_T, psec, wstride_cx, tb, conj);
+        else
+            leaf_mma_inst<T, NRT, G, 4, 1>(g, sh, nz, st, x, psi, wstride_T, psec, wstride_cx, tb, conj);
+    } else {
+        if (NRT == 1 && ct == 4)
+            leaf_mma_inst<T, NRT, G, 8, (NRT == 1 ? 4 : 1)>(g, sh, nz, st, x, psi, wstride_T, psec, wstride_cx, tb, conj);
+        else if (NRT <= 2 && ct >= 2)
+            leaf_mma_inst<T, NRT, G, 8, (NRT <= 2 ? 2 : 1)>(g, sh, nz, st, x, psi, wstride_T, psec, wstride_cx, tb, conj);
+        else
+            leaf_mma_inst<T, NRT, G, 8, 1>(g, sh, nz, st, x, psi, wstride_T, psec, wstride_cx, tb, conj);
+    }
 }
 
 template <typename T, int NRT>

@@ -325,10 +325,12 @@ __global__ __launch_bounds__(256) void k_leaf(Geom g, const cx<T> *__restrict__ 
 // i-1 by adding the difference of the two evaluations (two b-long kernel dot
 // products and two L2T rows), in full for sc > P/2 and halved for sc = P/2.
 //
-// Tile: 8 boxes x G sections (one section per wave at a time).  The kernel
+// Tile: 8 CT boxes x G sections (one section per wave at a time; CT column
+// tiles of 8 boxes share every A value, CT = 2 or 4 for leaf 32 or 16 so
+// that a tile carries as many MFMAs as one of leaf 64).  The kernel
 // is persistent: a workgroup loops over tiles and issues all global loads of
-// its next tile (x window of boxes i0-2 .. i0+8, kernel samples, local
-// expansions of boxes i0-1 .. i0+7) into registers before computing the
+// its next tile (x window of boxes i0-2 .. i0+BB, kernel samples, local
+// expansions of boxes i0-1 .. i0+BB-1) into registers before computing the
 // current one, so HBM latency hides behind the MFMAs.  Tiles are ordered so
 // that the workgroups of one XCD share x lines.
 // MFMA lane layout (16x16x4): lane l supplies A[l%16][l/16] and B[l/16][l%16]
@@ -368,14 +370,14 @@ struct Mfma<double> {
     static __device__ inline int row(int kr, int r) { return kr + 4 * r; }
 };
 
-template <typename T, int NRT, int G, int KT>
+template <typename T, int NRT, int G, int KT, int CT>
 __global__ __launch_bounds__(256) void k_leaf_mma(Geom g, const cx<T> *__restrict__ x, int64_t x_stride,
                                                   const T *__restrict__ psi, int64_t tree_stride,
                                                   cx<T> *__restrict__ psec, int64_t psec_stride,
                                                   const T *__restrict__ evfrag, const T *__restrict__ evalm,
                                                   const T *__restrict__ evalmh, const T *__restrict__ kv, int conj) {
     using V4 = typename Mfma<T>::v4;
-    constexpr int BB = 8;            // boxes per tile = MFMA columns / 2
+    constexpr int BB = 8 * CT;       // boxes per tile: CT column tiles of 8 boxes x re/im
     constexpr int b = 16 * NRT;      // leaf size
     constexpr int ROWS = (BB + 3) * b;
     constexpr int XN = ROWS * G, XPER = (XN + 255) / 256;           // x window
@@ -400,7 +402,7 @@ __global__ __launch_bounds__(256) void k_leaf_mma(Geom g, const cx<T> *__restric
     // bank, so the B loads (lanes over 8 boxes x re/im x 4 rows) are free of
     // LDS bank conflicts
     auto boxpad = [](int q) { return (q & 3) + 8 * (q >> 2); };
-    constexpr int XPAD = ((BB + 2) & 3) + 8 * ((BB + 2) >> 2) + 1;
+    constexpr int XPAD = ((BB + 2) & 3) + 8 * ((BB + 2) >> 2) + 1;  // boxpad(BB + 2) + 1
     cx<T> *xw = reinterpret_cast<cx<T> *>(smem);           // [ROWS][G] + pads
     T *kvs = reinterpret_cast<T *>(xw + XN + XPAD);        // [G][4b]
     T *lsx = kvs + KVN;                                    // [BB+1][TP][G][2] (+2 per box)
@@ -511,9 +513,11 @@ __global__ __launch_bounds__(256) void k_leaf_mma(Geom g, const cx<T> *__restric
             for (int s2 = 0; s2 < KT; ++s2)
 #pragma unroll
                 for (int rt = 0; rt < NRT; ++rt) la[rt][s2] = ef[(rt * KT + s2) << 6];
-            V4 acc[NRT];
+            V4 acc[NRT][CT];
 #pragma unroll
-            for (int rt = 0; rt < NRT; ++rt) acc[rt] = V4{0, 0, 0, 0};
+            for (int rt = 0; rt < NRT; ++rt)
+#pragma unroll
+                for (int ct = 0; ct < CT; ++ct) acc[rt][ct] = V4{0, 0, 0, 0};
             // near field.  K step s covers window row q = 4 s + kr of box bx
             // (rows (bx+1) b + q); A(rt, s) = K index 4 s + kr + b - 16 rt - col.
             // The Toeplitz structure gives A(rt+1, s+4) = A(rt, s), so each
@@ -524,77 +528,97 @@ __global__ __launch_bounds__(256) void k_leaf_mma(Geom g, const cx<T> *__restric
             for (int rt = 0; rt < NRT; ++rt)
 #pragma unroll
                 for (int t = 0; t < 4; ++t) areg[rt][t] = kr_[4 * t + kr + b - 16 * rt - col];
-            // window rows of group gi: (bx+1) b + 16 gi + 4 t + kr (+ box pad)
-            auto xrow = [&](int gi) {
-                return xwc + (((size_t)(bx + 1) * b + 16 * gi) * G + gg + boxpad(bx + 1 + gi / (b / 16))) * 2 + c;
+            // window rows of group gi for column tile ct (boxes 8 ct ..):
+            // (bx + 8 ct + 1) b + 16 gi + 4 t + kr (+ box pad)
+            auto xrow = [&](int gi, int ct) {
+                const int q = bx + 8 * ct + 1;
+                return xwc + (((size_t)q * b + 16 * gi) * G + gg + boxpad(q + gi / (b / 16))) * 2 + c;
             };
-            // B loads run one group ahead of the MFMAs that use them
-            T bvc[4];
-            {
-                const T *xg = xrow(0);
+            // B loads run one group ahead of the MFMAs that use them; the A
+            // values serve all CT column tiles
+            T bvc[CT][4];
 #pragma unroll
-                for (int t = 0; t < 4; ++t) bvc[t] = xg[(size_t)(4 * t + kr) * G * 2];
+            for (int ct = 0; ct < CT; ++ct) {
+                const T *xg = xrow(0, ct);
+#pragma unroll
+                for (int t = 0; t < 4; ++t) bvc[ct][t] = xg[(size_t)(4 * t + kr) * G * 2];
             }
 #pragma unroll
             for (int gi = 0; gi < NG; ++gi) {
-                T bvn[4], an[4];
+                T bvn[CT][4], an[4];
                 if (gi + 1 < NG) {
-                    const T *xg = xrow(gi + 1);
 #pragma unroll
-                    for (int t = 0; t < 4; ++t) {
-                        bvn[t] = xg[(size_t)(4 * t + kr) * G * 2];
-                        an[t] = kr_[16 * (gi + 1) + 4 * t + kr + b - col];
+                    for (int ct = 0; ct < CT; ++ct) {
+                        const T *xg = xrow(gi + 1, ct);
+#pragma unroll
+                        for (int t = 0; t < 4; ++t) bvn[ct][t] = xg[(size_t)(4 * t + kr) * G * 2];
                     }
+#pragma unroll
+                    for (int t = 0; t < 4; ++t) an[t] = kr_[16 * (gi + 1) + 4 * t + kr + b - col];
                 }
                 // keep the next group's loads ahead of this group's MFMAs
                 __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
                 for (int t = 0; t < 4; ++t)
 #pragma unroll
-                    for (int rt = 0; rt < NRT; ++rt) acc[rt] = Mfma<T>::mma(areg[rt][t], bvc[t], acc[rt]);
+                    for (int rt = 0; rt < NRT; ++rt)
+#pragma unroll
+                        for (int ct = 0; ct < CT; ++ct)
+                            acc[rt][ct] = Mfma<T>::mma(areg[rt][t], bvc[ct][t], acc[rt][ct]);
                 if (gi + 1 < NG) {
 #pragma unroll
                     for (int t = 0; t < 4; ++t) {
 #pragma unroll
                         for (int rt = NRT - 1; rt > 0; --rt) areg[rt][t] = areg[rt - 1][t];
                         areg[0][t] = an[t];
-                        bvc[t] = bvn[t];
+#pragma unroll
+                        for (int ct = 0; ct < CT; ++ct) bvc[ct][t] = bvn[ct][t];
                     }
                 }
             }
-            // far field (L2T): B = local expansion of box bx from LDS
-            const T *lb = lsx + (bx + 1) * LSTR + 2 * gg + c;
+            // far field (L2T): B = local expansion of box bx + 8 ct from LDS
 #pragma unroll
             for (int s2 = 0; s2 < KT; ++s2)
                 if (s2 < kt) {
-                    const T bvl = lb[(4 * s2 + kr) * 2 * G];
 #pragma unroll
-                    for (int rt = 0; rt < NRT; ++rt) acc[rt] = Mfma<T>::mma(la[rt][s2], bvl, acc[rt]);
+                    for (int ct = 0; ct < CT; ++ct) {
+                        const T bvl = lsx[(bx + 8 * ct + 1) * LSTR + 2 * gg + c + (4 * s2 + kr) * 2 * G];
+#pragma unroll
+                        for (int rt = 0; rt < NRT; ++rt) acc[rt][ct] = Mfma<T>::mma(la[rt][s2], bvl, acc[rt][ct]);
+                    }
                 }
             if (2 * sc >= P) {
-                // re-home target 0 of box bx to box bx-1: lanes col, col+16,
-                // col+32, col+48 each take a quarter of the correction terms
-                T corr = 0;
-                const T *xm2 = xwc + ((size_t)bx * b * G + gg + boxpad(bx)) * 2 + c;              // box i-2
-                const T *xp1 = xwc + ((size_t)(bx + 3) * b * G + gg + boxpad(bx + 3)) * 2 + c;    // box i+1
-                for (int t = kr; t < b; t += 4)
-                    corr += kr_[t] * xm2[(size_t)t * G * 2] - kr_[3 * b + t] * xp1[(size_t)t * G * 2];
-                const T *lm = lsx + bx * LSTR + 2 * gg + c;  // box i-1
+                // re-home target 0 of box q = bx + 8 ct to box q - 1: lanes
+                // col, col+16, col+32, col+48 each take a quarter of the terms
                 const T *eh = evalmh + (size_t)(sc - P / 2) * terms;
                 const T *e0 = evalm + (size_t)(sc - 1) * b * terms;  // row of target 0
-                for (int kk = kr; kk < terms; kk += 4) corr += eh[kk] * lm[kk * 2 * G] - e0[kk] * lb[kk * 2 * G];
-                corr += __shfl_xor(corr, 16, 64);
-                corr += __shfl_xor(corr, 32, 64);
-                if (kr == 0) acc[0][0] += (2 * sc == P) ? T(0.5) * corr : corr;
+#pragma unroll
+                for (int ct = 0; ct < CT; ++ct) {
+                    const int q = bx + 8 * ct;
+                    T corr = 0;
+                    const T *xm2 = xwc + ((size_t)q * b * G + gg + boxpad(q)) * 2 + c;              // box i-2
+                    const T *xp1 = xwc + ((size_t)(q + 3) * b * G + gg + boxpad(q + 3)) * 2 + c;    // box i+1
+                    for (int t = kr; t < b; t += 4)
+                        corr += kr_[t] * xm2[(size_t)t * G * 2] - kr_[3 * b + t] * xp1[(size_t)t * G * 2];
+                    const T *lm = lsx + q * LSTR + 2 * gg + c;        // box i-1
+                    const T *lb = lsx + (q + 1) * LSTR + 2 * gg + c;  // box i
+                    for (int kk = kr; kk < terms; kk += 4)
+                        corr += eh[kk] * lm[kk * 2 * G] - e0[kk] * lb[kk * 2 * G];
+                    corr += __shfl_xor(corr, 16, 64);
+                    corr += __shfl_xor(corr, 32, 64);
+                    if (kr == 0) acc[0][ct][0] += (2 * sc == P) ? T(0.5) * corr : corr;
+                }
             }
             // write: pair lanes (c = 0, 1) swap halves so that each lane holds
             // two complete complex outputs; 8 lanes cover 16 consecutive targets
             // (bit selects: a ?: between vector elements becomes an indexed
             // extract over all accumulators, a long compare/select chain)
-            cx<T> *dst = psec + (int64_t)sc * M + (int64_t)(i0 + bx) * b;
+#pragma unroll
+            for (int ct = 0; ct < CT; ++ct)
 #pragma unroll
             for (int rt = 0; rt < NRT; ++rt) {
-                const T a0 = acc[rt][0], a1 = acc[rt][1], a2 = acc[rt][2], a3 = acc[rt][3];
+                cx<T> *dst = psec + (int64_t)sc * M + (int64_t)(i0 + bx + 8 * ct) * b;
+                const T a0 = acc[rt][ct][0], a1 = acc[rt][ct][1], a2 = acc[rt][ct][2], a3 = acc[rt][ct][3];
                 const T keep0 = bsel(c, a2, a0), keep1 = bsel(c, a3, a1);
                 const T recv0 = __shfl_xor(bsel(c, a0, a2), 1, 64);
                 const T recv1 = __shfl_xor(bsel(c, a1, a3), 1, 64);
@@ -751,12 +775,17 @@ __global__ __launch_bounds__(256) void k_tree_mma(Geom g, int l, const T *__rest
 // ---------------------------------------------------------------------------
 constexpr int kCombineThreads = 256;
 
+// Rows mu per combine tile: 2048 elements per tile for every P (16 KB of
+// complex64 read), so small P keeps as many bytes in flight as P = 64.
+template <int P>
+constexpr int combine_rows() { return 2048 / P; }
+
 template <typename T, int P>
 __global__ __launch_bounds__(kCombineThreads) void k_combine(Geom g, const cx<T> *__restrict__ psec, int64_t psec_stride,
                                                  const c64 *__restrict__ sumsec, int64_t sumsec_stride,
                                                  const c64 *__restrict__ gfac, const c64 *__restrict__ wP,
                                                  cx<T> *__restrict__ y, int64_t y_stride) {
-    constexpr int RB = 32;
+    constexpr int RB = combine_rows<P>();
     constexpr int NT = kCombineThreads;
     constexpr int ls = row_stride<P>();
     constexpr int PER = (RB * P + NT - 1) / NT;
