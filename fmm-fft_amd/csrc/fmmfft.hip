@@ -155,8 +155,9 @@ int leaf_ct(const Geom &g) {
 size_t leaf_mma_lds(const Geom &g, size_t esize, int G) {
     const int bb = 8 * leaf_ct(g);
     const int q = bb + 2, xpad = (q & 3) + 8 * (q >> 2) + 1;
+    const size_t tp = 4 * (size_t)leaf_kt(g.terms);
     return ((size_t)(bb + 3) * g.b * G + xpad) * 2 * esize + (size_t)G * 4 * g.b * esize +
-           (size_t)(bb + 1) * (4 * leaf_kt(g.terms) * 2 * G + 2) * esize;
+           (size_t)(bb + 1) * (tp * 2 * G + 2) * esize + (size_t)G * 2 * tp * esize;
 }
 // (the persistent leaf kernel's LDS is at most 11*128*4*16 + ... < 100 KB)
 
@@ -339,8 +340,12 @@ void leaf_mma_inst(const Geom &g, const Shapes &sh, unsigned nz, hipStream_t st,
         (void)hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh.leaf_lds);
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 256, sh.leaf_lds) != hipSuccess || per_cu < 1)
         per_cu = 1;
-    const int64_t ntiles = (int64_t)(g.P / G) * (g.s / (8 * CT));
-    const unsigned nwg = (unsigned)std::min<int64_t>(ntiles, (int64_t)std::max(cus, 8) * per_cu);
+    // a multiple of the section groups: each workgroup keeps one group
+    const int64_t ngroups = g.P / G;
+    const int64_t ntiles = ngroups * (g.s / (8 * CT));
+    int64_t nwg64 = std::min<int64_t>(ntiles, (int64_t)std::max(cus, 8) * per_cu);
+    nwg64 = std::max<int64_t>(ngroups, nwg64 / ngroups * ngroups);
+    const unsigned nwg = (unsigned)nwg64;
     hipLaunchKernelGGL(kern, dim3(nwg, 1, nz), dim3(256), sh.leaf_lds, st, g, x, g.N, psi, wstride_T, psec,
                        wstride_cx, tb.evfrag, tb.evalm, tb.evalmh, tb.kv, conj);
 }

@@ -349,6 +349,17 @@ __device__ inline double bsel(int p, double a, double b) {
                                             ((unsigned long long)__double_as_longlong(b) & ~m)));
 }
 
+// value of lane l ^ 1 (a DPP quad permutation [1,0,3,2]; no LDS round trip)
+__device__ inline float swap_pair(float v) {
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xB1, 0xF, 0xF, true));
+}
+__device__ inline double swap_pair(double v) {
+    const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_mov_dpp((int)(unsigned)u, 0xB1, 0xF, 0xF, true);
+    const unsigned hi = (unsigned)__builtin_amdgcn_mov_dpp((int)(unsigned)(u >> 32), 0xB1, 0xF, 0xF, true);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
 template <typename T>
 struct Mfma;
 // Output row of accumulator element r held by lanes with l/16 = kr (both
@@ -370,8 +381,19 @@ struct Mfma<double> {
     static __device__ inline int row(int kr, int r) { return kr + 4 * r; }
 };
 
+// Waves per SIMD the register allocation of a leaf instance is held to:
+// three for the FP32 leaf-64 instances, which fit 168 registers without
+// spilling (left alone they take ~180 and run two workgroups per CU).
 template <typename T, int NRT, int G, int KT, int CT>
-__global__ __launch_bounds__(256) void k_leaf_mma(Geom g, const cx<T> *__restrict__ x, int64_t x_stride,
+constexpr int leaf_waves() {
+    if (sizeof(T) == 4 && G == 4 && NRT == 4 && CT == 1) return 3;
+    if (sizeof(T) == 8 && G == 4 && NRT == 4 && CT == 1 && KT == 4) return 2;
+    return 1;
+}
+
+template <typename T, int NRT, int G, int KT, int CT>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(leaf_waves<T, NRT, G, KT, CT>())))
+void k_leaf_mma(Geom g, const cx<T> *__restrict__ x, int64_t x_stride,
                                                   const T *__restrict__ psi, int64_t tree_stride,
                                                   cx<T> *__restrict__ psec, int64_t psec_stride,
                                                   const T *__restrict__ evfrag, const T *__restrict__ evalm,
@@ -381,7 +403,7 @@ __global__ __launch_bounds__(256) void k_leaf_mma(Geom g, const cx<T> *__restric
     constexpr int b = 16 * NRT;      // leaf size
     constexpr int ROWS = (BB + 3) * b;
     constexpr int XN = ROWS * G, XPER = (XN + 255) / 256;           // x window
-    constexpr int KVN = G * 4 * b, KPER = (KVN + 255) / 256;        // kernel samples
+    constexpr int KVN = G * 4 * b;                                   // kernel samples
     // local expansions in LDS with the terms padded to TP = 4 KT (the padding
     // slots hold copies of the last term and meet zero L2T rows), so every
     // index below is a compile-time shift or mask
@@ -389,11 +411,13 @@ __global__ __launch_bounds__(256) void k_leaf_mma(Geom g, const cx<T> *__restric
     constexpr int NL = TP * 2 * G;                                   // one box
     constexpr int LSTR = NL + 2;                                     // +2: bank shift
     constexpr int LN = (BB + 1) * NL, LPER = (LN + 255) / 256;
+    // evaluation rows of the target-0 correction: [G][2][TP] (row of the
+    // re-homed target, row of target 0; zero past terms)
+    constexpr int EN = G * 2 * TP;
     extern __shared__ __align__(16) unsigned char smem[];
     const int P = g.P, terms = g.terms, S = g.S;
     const int64_t M = g.M;
     const int rowsS = 2 * S;             // tree row stride
-    const int kt = (terms + 3) / 4;      // L2T K steps (<= KT)
     // offsets inside one signal fit 32 bits (N <= 2^26); P, M and s are
     // powers of two
     const int lp = __builtin_ctz(P), mmask = (int)M - 1, smask = g.s - 1;
@@ -406,6 +430,7 @@ __global__ __launch_bounds__(256) void k_leaf_mma(Geom g, const cx<T> *__restric
     cx<T> *xw = reinterpret_cast<cx<T> *>(smem);           // [ROWS][G] + pads
     T *kvs = reinterpret_cast<T *>(xw + XN + XPAD);        // [G][4b]
     T *lsx = kvs + KVN;                                    // [BB+1][TP][G][2] (+2 per box)
+    T *evs = lsx + (BB + 1) * LSTR;                        // [G][2][TP]
     const int sig = blockIdx.z;
     x += (int64_t)sig * x_stride;
     psi += (int64_t)sig * tree_stride + level_off(g, g.n);
@@ -413,50 +438,72 @@ __global__ __launch_bounds__(256) void k_leaf_mma(Geom g, const cx<T> *__restric
 
     // Persistent tiles.  Iteration k of workgroup w takes tile k*nwg + xo(w);
     // xo makes the workgroups of one XCD take consecutive tiles (the section
-    // groups of one box chunk share x lines).
+    // groups of one box chunk share x lines).  The grid is a multiple of the
+    // number of section groups, so a workgroup keeps the section group
+    // xo % ngroups for all its tiles and loads the section constants once.
     const int ngroups = P / G;
     const int ntiles = ngroups * (g.s / BB);
     const int nwg = gridDim.x, w = blockIdx.x;
     const int xo = (nwg % 8 == 0) ? (w % 8) * (nwg / 8) + w / 8 : w;
+    const int sc0 = (xo % ngroups) * G;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int col = lane & 15, kr = lane >> 4;
+    const int bx = col >> 1, c = col & 1;
+    const T *xwc = reinterpret_cast<const T *>(xw);
+    static_assert(G <= 4, "one section per wave");
+    const int gg = wave, sc = sc0 + gg;
+
+    // Section constants (clamped section index: section 0 and the waves past
+    // G = 2 load a valid row they never use): kernel samples [G][4b] and the
+    // target-0 correction rows [G][2][TP] (zero past terms) in LDS, and this
+    // wave's L2T A fragments (lane order, KT steps per row tile, zero past
+    // terms) in registers.  The first tile's barriers publish the LDS part.
+    for (int e = threadIdx.x; e < KVN; e += 256) {
+        const int sce = min(max(sc0 + e / (4 * b), 1), P - 1);
+        kvs[e] = kv[(sce - 1) * 4 * b + e % (4 * b)];
+    }
+    for (int e = threadIdx.x; e < EN; e += 256) {
+        const int sce = min(max(sc0 + e / (2 * TP), 1), P - 1), kk = e % TP;
+        const T *row = ((e / TP) & 1) ? evalm + (size_t)(sce - 1) * b * terms
+                                      : evalmh + (size_t)max(sce - P / 2, 0) * terms;
+        const T v = row[min(kk, terms - 1)];
+        evs[e] = kk < terms ? v : T(0);
+    }
+    T la[NRT][KT];
+    {
+        const int scl = min(max(sc, 1), P - 1);
+        const T *ef = evfrag + (((scl - 1) * NRT * KT) << 6) + lane;
+#pragma unroll
+        for (int s2 = 0; s2 < KT; ++s2)
+#pragma unroll
+            for (int rt = 0; rt < NRT; ++rt) la[rt][s2] = ef[(rt * KT + s2) << 6];
+    }
 
     cx<T> xr[XPER];
-    T kvr[KPER], lr[LPER];
+    T lr[LPER];
     // issue every global load of a tile into registers (they stay in flight
-    // while the previous tile computes)
+    // while the previous tile computes).  Every load is unconditional
+    // (indices clamped): a lane-conditional load becomes a branch, and the
+    // counter wait after a branch covers every load in flight.
     auto prefetch = [&](int lid) __attribute__((always_inline)) {
-        const int sc0 = (lid % ngroups) * G, i0 = (lid / ngroups) * BB;
+        const int i0 = (lid / ngroups) * BB;
         const int mu_w0 = (i0 - 2) * b;
 #pragma unroll
         for (int u = 0; u < XPER; ++u) {
-            const int e = threadIdx.x + 256 * u;
-            if (XN % 256 == 0 || e < XN) {
-                const int r = e / G, gg = e % G;
-                const int mu = (mu_w0 + r) & mmask;  // periodic window
-                xr[u] = x[(mu << lp) + sc0 + gg];
-            }
-        }
-#pragma unroll
-        for (int u = 0; u < KPER; ++u) {
-            const int e = threadIdx.x + 256 * u;
-            if (KVN % 256 == 0 || e < KVN) {
-                const int gg = e / (4 * b), d = e % (4 * b);
-                const int sc = sc0 + gg;
-                kvr[u] = kv[max(sc - 1, 0) * 4 * b + d];  // section 0 never reads it
-            }
+            const int e = (XN % 256 == 0) ? threadIdx.x + 256 * u : min((int)threadIdx.x + 256 * u, XN - 1);
+            const int r = e / G, ge = e % G;
+            const int mu = (mu_w0 + r) & mmask;  // periodic window
+            xr[u] = x[(mu << lp) + sc0 + ge];
         }
 #pragma unroll
         for (int u = 0; u < LPER; ++u) {
-            const int e = threadIdx.x + 256 * u;
-            if (LN % 256 == 0 || e < LN) {
-                const int bx9 = e / NL, rem = e % NL;
-                const int k = rem / (2 * G), rr = rem % (2 * G);
-                const int sc = sc0 + (rr >> 1);
-                const int box = (i0 - 1 + bx9) & smask;
-                // clamped, unconditional load (a lane-conditional load becomes
-                // a branch that waits for it): padding terms meet zero L2T
-                // rows, section 0 never reads its slot
-                lr[u] = psi[(box * terms + min(k, terms - 1)) * rowsS + max(2 * (sc - 1) + (rr & 1), 0)];
-            }
+            const int e = (LN % 256 == 0) ? threadIdx.x + 256 * u : min((int)threadIdx.x + 256 * u, LN - 1);
+            const int bx9 = e / NL, rem = e % NL;
+            const int kk = rem / (2 * G), rr = rem % (2 * G);
+            const int sce = sc0 + (rr >> 1);
+            const int box = (i0 - 1 + bx9) & smask;
+            // padding terms meet zero L2T rows, section 0 never reads its slot
+            lr[u] = psi[(box * terms + min(kk, terms - 1)) * rowsS + max(2 * (sce - 1) + (rr & 1), 0)];
         }
     };
     auto commit = [&]() __attribute__((always_inline)) {
@@ -470,21 +517,11 @@ __global__ __launch_bounds__(256) void k_leaf_mma(Geom g, const cx<T> *__restric
             }
         }
 #pragma unroll
-        for (int u = 0; u < KPER; ++u) {
-            const int e = threadIdx.x + 256 * u;
-            if (KVN % 256 == 0 || e < KVN) kvs[e] = kvr[u];
-        }
-#pragma unroll
         for (int u = 0; u < LPER; ++u) {
             const int e = threadIdx.x + 256 * u;
             if (LN % 256 == 0 || e < LN) lsx[(e / NL) * LSTR + e % NL] = lr[u];
         }
     };
-
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int col = lane & 15, kr = lane >> 4;
-    const int bx = col >> 1, c = col & 1;
-    const T *xwc = reinterpret_cast<const T *>(xw);
 
     int k = 0;
     int lid = xo;
@@ -493,144 +530,138 @@ __global__ __launch_bounds__(256) void k_leaf_mma(Geom g, const cx<T> *__restric
         __syncthreads();  // the previous tile is no longer read
         commit();
         __syncthreads();
+        const int i0 = (lid / ngroups) * BB;
+        // the last iteration prefetches a valid tile it never uses (an
+        // unconditional prefetch keeps the counter waits exact)
         const int nlid = (++k) * nwg + xo;
-        if (nlid < ntiles) prefetch(nlid);
-        const int sc0 = (lid % ngroups) * G, i0 = (lid / ngroups) * BB;
+        prefetch(min(nlid, ntiles - 1));
 
-        for (int gg = wave; gg < G; gg += 4) {
-            const int sc = sc0 + gg;
+        if (gg < G) {
             if (sc == 0) {
                 cx<T> *dst = psec + (int64_t)i0 * b;
                 for (int t = lane; t < BB * b; t += 64) dst[t] = xw[(2 * b + t) * G + boxpad((2 * b + t) / b)];
-                continue;
-            }
-            const T *kr_ = kvs + gg * 4 * b;
-            // L2T A fragments, stored in lane order with KT steps per row tile
-            // (zero past terms): one coalesced load per step, constant offsets
-            T la[NRT][KT];
-            const T *ef = evfrag + (((sc - 1) * NRT * KT) << 6) + lane;
+            } else {
+                const T *kr_ = kvs + gg * 4 * b;
+                V4 acc[NRT][CT];
 #pragma unroll
-            for (int s2 = 0; s2 < KT; ++s2)
+                for (int rt = 0; rt < NRT; ++rt)
 #pragma unroll
-                for (int rt = 0; rt < NRT; ++rt) la[rt][s2] = ef[(rt * KT + s2) << 6];
-            V4 acc[NRT][CT];
+                    for (int ct = 0; ct < CT; ++ct) acc[rt][ct] = V4{0, 0, 0, 0};
+                // near field.  K step s covers window row q = 4 s + kr of box bx
+                // (rows (bx+1) b + q); A(rt, s) = K index 4 s + kr + b - 16 rt - col.
+                // The Toeplitz structure gives A(rt+1, s+4) = A(rt, s), so each
+                // group of four steps loads only the rt = 0 values and shifts the rest.
+                constexpr int NG = 3 * b / 16;  // groups of four K steps
+                T areg[NRT][4];
 #pragma unroll
-            for (int rt = 0; rt < NRT; ++rt)
+                for (int rt = 0; rt < NRT; ++rt)
 #pragma unroll
-                for (int ct = 0; ct < CT; ++ct) acc[rt][ct] = V4{0, 0, 0, 0};
-            // near field.  K step s covers window row q = 4 s + kr of box bx
-            // (rows (bx+1) b + q); A(rt, s) = K index 4 s + kr + b - 16 rt - col.
-            // The Toeplitz structure gives A(rt+1, s+4) = A(rt, s), so each
-            // group of four steps loads only the rt = 0 values and shifts the rest.
-            constexpr int NG = 3 * b / 16;  // groups of four K steps
-            T areg[NRT][4];
+                    for (int t = 0; t < 4; ++t) areg[rt][t] = kr_[4 * t + kr + b - 16 * rt - col];
+                // window rows of group gi for column tile ct (boxes 8 ct ..):
+                // (bx + 8 ct + 1) b + 16 gi + 4 t + kr (+ box pad)
+                auto xrow = [&](int gi, int ct) {
+                    const int q = bx + 8 * ct + 1;
+                    return xwc + (((size_t)q * b + 16 * gi) * G + gg + boxpad(q + gi / (b / 16))) * 2 + c;
+                };
+                // B loads run one group ahead of the MFMAs that use them; the A
+                // values serve all CT column tiles
+                T bvc[CT][4];
 #pragma unroll
-            for (int rt = 0; rt < NRT; ++rt)
+                for (int ct = 0; ct < CT; ++ct) {
+                    const T *xg = xrow(0, ct);
 #pragma unroll
-                for (int t = 0; t < 4; ++t) areg[rt][t] = kr_[4 * t + kr + b - 16 * rt - col];
-            // window rows of group gi for column tile ct (boxes 8 ct ..):
-            // (bx + 8 ct + 1) b + 16 gi + 4 t + kr (+ box pad)
-            auto xrow = [&](int gi, int ct) {
-                const int q = bx + 8 * ct + 1;
-                return xwc + (((size_t)q * b + 16 * gi) * G + gg + boxpad(q + gi / (b / 16))) * 2 + c;
-            };
-            // B loads run one group ahead of the MFMAs that use them; the A
-            // values serve all CT column tiles
-            T bvc[CT][4];
-#pragma unroll
-            for (int ct = 0; ct < CT; ++ct) {
-                const T *xg = xrow(0, ct);
-#pragma unroll
-                for (int t = 0; t < 4; ++t) bvc[ct][t] = xg[(size_t)(4 * t + kr) * G * 2];
-            }
-#pragma unroll
-            for (int gi = 0; gi < NG; ++gi) {
-                T bvn[CT][4], an[4];
-                if (gi + 1 < NG) {
-#pragma unroll
-                    for (int ct = 0; ct < CT; ++ct) {
-                        const T *xg = xrow(gi + 1, ct);
-#pragma unroll
-                        for (int t = 0; t < 4; ++t) bvn[ct][t] = xg[(size_t)(4 * t + kr) * G * 2];
-                    }
-#pragma unroll
-                    for (int t = 0; t < 4; ++t) an[t] = kr_[16 * (gi + 1) + 4 * t + kr + b - col];
+                    for (int t = 0; t < 4; ++t) bvc[ct][t] = xg[(size_t)(4 * t + kr) * G * 2];
                 }
-                // keep the next group's loads ahead of this group's MFMAs
-                __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-                for (int t = 0; t < 4; ++t)
+                for (int gi = 0; gi < NG; ++gi) {
+                    T bvn[CT][4], an[4];
+                    if (gi + 1 < NG) {
 #pragma unroll
-                    for (int rt = 0; rt < NRT; ++rt)
+                        for (int ct = 0; ct < CT; ++ct) {
+                            const T *xg = xrow(gi + 1, ct);
 #pragma unroll
-                        for (int ct = 0; ct < CT; ++ct)
-                            acc[rt][ct] = Mfma<T>::mma(areg[rt][t], bvc[ct][t], acc[rt][ct]);
-                if (gi + 1 < NG) {
+                            for (int t = 0; t < 4; ++t) bvn[ct][t] = xg[(size_t)(4 * t + kr) * G * 2];
+                        }
 #pragma unroll
-                    for (int t = 0; t < 4; ++t) {
+                        for (int t = 0; t < 4; ++t) an[t] = kr_[16 * (gi + 1) + 4 * t + kr + b - col];
+                    }
+                    // keep the next group's loads ahead of this group's MFMAs
+                    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-                        for (int rt = NRT - 1; rt > 0; --rt) areg[rt][t] = areg[rt - 1][t];
-                        areg[0][t] = an[t];
+                    for (int t = 0; t < 4; ++t)
 #pragma unroll
-                        for (int ct = 0; ct < CT; ++ct) bvc[ct][t] = bvn[ct][t];
+                        for (int rt = 0; rt < NRT; ++rt)
+#pragma unroll
+                            for (int ct = 0; ct < CT; ++ct)
+                                acc[rt][ct] = Mfma<T>::mma(areg[rt][t], bvc[ct][t], acc[rt][ct]);
+                    if (gi + 1 < NG) {
+#pragma unroll
+                        for (int t = 0; t < 4; ++t) {
+#pragma unroll
+                            for (int rt = NRT - 1; rt > 0; --rt) areg[rt][t] = areg[rt - 1][t];
+                            areg[0][t] = an[t];
+#pragma unroll
+                            for (int ct = 0; ct < CT; ++ct) bvc[ct][t] = bvn[ct][t];
+                        }
                     }
                 }
-            }
-            // far field (L2T): B = local expansion of box bx + 8 ct from LDS
+                // far field (L2T): B = local expansion of box bx + 8 ct from LDS
 #pragma unroll
-            for (int s2 = 0; s2 < KT; ++s2)
-                if (s2 < kt) {
+                for (int s2 = 0; s2 < KT; ++s2)
 #pragma unroll
                     for (int ct = 0; ct < CT; ++ct) {
                         const T bvl = lsx[(bx + 8 * ct + 1) * LSTR + 2 * gg + c + (4 * s2 + kr) * 2 * G];
 #pragma unroll
                         for (int rt = 0; rt < NRT; ++rt) acc[rt][ct] = Mfma<T>::mma(la[rt][s2], bvl, acc[rt][ct]);
                     }
+                if (2 * sc >= P) {
+                    // re-home target 0 of box q = bx + 8 ct to box q - 1: lanes
+                    // col, col+16, col+32, col+48 each take a quarter of the terms
+#pragma unroll
+                    for (int ct = 0; ct < CT; ++ct) {
+                        const int q = bx + 8 * ct;
+                        T corr = 0;
+                        const T *xm2 = xwc + ((size_t)q * b * G + gg + boxpad(q)) * 2 + c;              // box i-2
+                        const T *xp1 = xwc + ((size_t)(q + 3) * b * G + gg + boxpad(q + 3)) * 2 + c;    // box i+1
+#pragma unroll 4
+                        for (int j = 0; j < b / 4; ++j) {
+                            const int t = 4 * j + kr;
+                            corr += kr_[t] * xm2[(size_t)t * G * 2] - kr_[3 * b + t] * xp1[(size_t)t * G * 2];
+                        }
+                        const T *lm = lsx + q * LSTR + 2 * gg + c;        // box i-1
+                        const T *lb = lsx + (q + 1) * LSTR + 2 * gg + c;  // box i
+                        const T *eh = evs + gg * 2 * TP, *e0 = eh + TP;
+#pragma unroll
+                        for (int j = 0; j < KT; ++j)
+                            corr += eh[4 * j + kr] * lm[(4 * j + kr) * 2 * G] - e0[4 * j + kr] * lb[(4 * j + kr) * 2 * G];
+                        corr += __shfl_xor(corr, 16, 64);
+                        corr += __shfl_xor(corr, 32, 64);
+                        if (kr == 0) acc[0][ct][0] += (2 * sc == P) ? T(0.5) * corr : corr;
+                    }
                 }
-            if (2 * sc >= P) {
-                // re-home target 0 of box q = bx + 8 ct to box q - 1: lanes
-                // col, col+16, col+32, col+48 each take a quarter of the terms
-                const T *eh = evalmh + (size_t)(sc - P / 2) * terms;
-                const T *e0 = evalm + (size_t)(sc - 1) * b * terms;  // row of target 0
+                // write: pair lanes (c = 0, 1) swap halves so that each lane holds
+                // two complete complex outputs; 8 lanes cover 16 consecutive targets
+                // (bit selects: a ?: between vector elements becomes an indexed
+                // extract over all accumulators, a long compare/select chain)
 #pragma unroll
-                for (int ct = 0; ct < CT; ++ct) {
-                    const int q = bx + 8 * ct;
-                    T corr = 0;
-                    const T *xm2 = xwc + ((size_t)q * b * G + gg + boxpad(q)) * 2 + c;              // box i-2
-                    const T *xp1 = xwc + ((size_t)(q + 3) * b * G + gg + boxpad(q + 3)) * 2 + c;    // box i+1
-                    for (int t = kr; t < b; t += 4)
-                        corr += kr_[t] * xm2[(size_t)t * G * 2] - kr_[3 * b + t] * xp1[(size_t)t * G * 2];
-                    const T *lm = lsx + q * LSTR + 2 * gg + c;        // box i-1
-                    const T *lb = lsx + (q + 1) * LSTR + 2 * gg + c;  // box i
-                    for (int kk = kr; kk < terms; kk += 4)
-                        corr += eh[kk] * lm[kk * 2 * G] - e0[kk] * lb[kk * 2 * G];
-                    corr += __shfl_xor(corr, 16, 64);
-                    corr += __shfl_xor(corr, 32, 64);
-                    if (kr == 0) acc[0][ct][0] += (2 * sc == P) ? T(0.5) * corr : corr;
-                }
-            }
-            // write: pair lanes (c = 0, 1) swap halves so that each lane holds
-            // two complete complex outputs; 8 lanes cover 16 consecutive targets
-            // (bit selects: a ?: between vector elements becomes an indexed
-            // extract over all accumulators, a long compare/select chain)
+                for (int ct = 0; ct < CT; ++ct)
 #pragma unroll
-            for (int ct = 0; ct < CT; ++ct)
-#pragma unroll
-            for (int rt = 0; rt < NRT; ++rt) {
-                cx<T> *dst = psec + (int64_t)sc * M + (int64_t)(i0 + bx + 8 * ct) * b;
-                const T a0 = acc[rt][ct][0], a1 = acc[rt][ct][1], a2 = acc[rt][ct][2], a3 = acc[rt][ct][3];
-                const T keep0 = bsel(c, a2, a0), keep1 = bsel(c, a3, a1);
-                const T recv0 = __shfl_xor(bsel(c, a0, a2), 1, 64);
-                const T recv1 = __shfl_xor(bsel(c, a1, a3), 1, 64);
-                const cx<T> o0{bsel(c, recv0, keep0), bsel(c, keep0, recv0)};
-                const cx<T> o1{bsel(c, recv1, keep1), bsel(c, keep1, recv1)};
-                if constexpr (sizeof(T) == 4) {
-                    // rows 4 kr + 2c, 4 kr + 2c + 1: one 16-byte store
-                    *reinterpret_cast<float4 *>(dst + 16 * rt + Mfma<T>::row(kr, 2 * c)) =
-                        make_float4(o0.re, o0.im, o1.re, o1.im);
-                } else {
-                    dst[16 * rt + Mfma<T>::row(kr, 2 * c)] = o0;
-                    dst[16 * rt + Mfma<T>::row(kr, 2 * c + 1)] = o1;
+                for (int rt = 0; rt < NRT; ++rt) {
+                    cx<T> *dst = psec + (int64_t)sc * M + (int64_t)(i0 + bx + 8 * ct) * b;
+                    const T a0 = acc[rt][ct][0], a1 = acc[rt][ct][1], a2 = acc[rt][ct][2], a3 = acc[rt][ct][3];
+                    const T keep0 = bsel(c, a2, a0), keep1 = bsel(c, a3, a1);
+                    const T recv0 = swap_pair(bsel(c, a0, a2));
+                    const T recv1 = swap_pair(bsel(c, a1, a3));
+                    const cx<T> o0{bsel(c, recv0, keep0), bsel(c, keep0, recv0)};
+                    const cx<T> o1{bsel(c, recv1, keep1), bsel(c, keep1, recv1)};
+                    if constexpr (sizeof(T) == 4) {
+                        // rows 4 kr + 2c, 4 kr + 2c + 1: one 16-byte store
+                        *reinterpret_cast<float4 *>(dst + 16 * rt + Mfma<T>::row(kr, 2 * c)) =
+                            make_float4(o0.re, o0.im, o1.re, o1.im);
+                    } else {
+                        dst[16 * rt + Mfma<T>::row(kr, 2 * c)] = o0;
+                        dst[16 * rt + Mfma<T>::row(kr, 2 * c + 1)] = o1;
+                    }
                 }
             }
         }
