@@ -132,6 +132,7 @@ TableLayout table_layout(const Geom &g, const FftLayout &f) {
 struct Shapes {
     int s2m_bpb, s2m_nblk;
     size_t s2m_lds;
+    int s2m_vec;  // matrix-core S2M: reals per lane load (0: vector-FMA kernel)
     int leaf_G, leaf_bb;
     size_t leaf_lds;
     bool leaf_mma;  // matrix-core leaf kernel (leaf size a multiple of 16)
@@ -170,6 +171,24 @@ Shapes shapes(const Geom &g, size_t esize) {
     sh.s2m_bpb = s2m_bx;
     sh.s2m_nblk = std::min(g.s / s2m_bx, 512);
     sh.s2m_lds = (size_t)g.b * padded_terms(g.terms) * esize;
+    // matrix-core S2M for complex64, leaf 16..64: a row of x (2P reals)
+    // splits into chunks of 16 VEC reals, VEC = 4 (P >= 32) or 2 (P = 16);
+    // one chunk per wave, 4 / chunks boxes per workgroup step, at most 768
+    // workgroups (three per CU: the leaf-64 instances hold 3 to 5 per CU).
+    // complex128 keeps the vector-FMA kernel, which already streams x at
+    // copy speed (the FP64 VALU rate equals the FP32 one here).
+    {
+        int vec = 0;
+        if (esize == 4 && g.b % 16 == 0 && g.b <= 64) {
+            if (2 * g.P >= 64) vec = 4;
+            else if (2 * g.P >= 32) vec = 2;
+        }
+        sh.s2m_vec = vec;
+        if (vec) {
+            const int nchunk = 2 * g.P / (16 * vec), bper = 4 / nchunk;
+            sh.s2m_nblk = std::min((g.s + bper - 1) / bper, 768);
+        }
+    }
 
     int G = std::min(g.P, esize == 4 ? 16 : 8);
     int bb = std::min(g.s, 4);
@@ -392,12 +411,50 @@ void launch_leaf_mma(const Geom &g, const Shapes &sh, unsigned nz, hipStream_t s
     }
 }
 
+template <typename T, int VEC, int MT>
+void s2m_mma_ks(const Geom &g, const Shapes &sh, unsigned nz, hipStream_t st, const cx<T> *x, T *phi,
+                int64_t wstride_T, c64 *sums_part, int64_t wstride_c64, const T *initch, int conj) {
+    const dim3 grid(sh.s2m_nblk, 1, nz);
+    switch (g.b) {
+        case 16:
+            hipLaunchKernelGGL((k_s2m_mma<T, VEC, MT, 4>), grid, dim3(256), 0, st, g, x, g.N, phi, wstride_T,
+                               sums_part, wstride_c64, initch, conj);
+            break;
+        case 32:
+            hipLaunchKernelGGL((k_s2m_mma<T, VEC, MT, 8>), grid, dim3(256), 0, st, g, x, g.N, phi, wstride_T,
+                               sums_part, wstride_c64, initch, conj);
+            break;
+        default:
+            hipLaunchKernelGGL((k_s2m_mma<T, VEC, MT, 16>), grid, dim3(256), 0, st, g, x, g.N, phi, wstride_T,
+                               sums_part, wstride_c64, initch, conj);
+            break;
+    }
+}
+
+// matrix-core S2M (complex64): VEC from the plan's shapes, MT = 16-term blocks
+template <typename T>
+void launch_s2m_mma(const Geom &g, const Shapes &sh, unsigned nz, hipStream_t st, const cx<T> *x, T *phi,
+                    int64_t wstride_T, c64 *sums_part, int64_t wstride_c64, const T *initch, int conj) {
+    if constexpr (sizeof(T) == 4) {
+        if (sh.s2m_vec == 4) {
+            if (g.terms <= 16) s2m_mma_ks<T, 4, 1>(g, sh, nz, st, x, phi, wstride_T, sums_part, wstride_c64, initch, conj);
+            else s2m_mma_ks<T, 4, 2>(g, sh, nz, st, x, phi, wstride_T, sums_part, wstride_c64, initch, conj);
+        } else {
+            if (g.terms <= 16) s2m_mma_ks<T, 2, 1>(g, sh, nz, st, x, phi, wstride_T, sums_part, wstride_c64, initch, conj);
+            else s2m_mma_ks<T, 2, 2>(g, sh, nz, st, x, phi, wstride_T, sums_part, wstride_c64, initch, conj);
+        }
+    }
+}
+
 template <typename T, int PT, typename Mark>
 void s2m_tree_pt(const Geom &g, const Shapes &sh, unsigned nz, hipStream_t st, const cx<T> *x, T *phi, T *psi,
                  int64_t wstride_T, c64 *sums_part, int64_t wstride_c64, c64 *sumsec, const TabPtrs<T> &tb,
                  int conj, Mark &mark) {
-    hipLaunchKernelGGL((k_s2m<T, PT>), dim3(sh.s2m_nblk, 1, nz), dim3(kS2mThreads), sh.s2m_lds, st, g, x, g.N,
-                       phi, wstride_T, sums_part, wstride_c64, tb.initch, conj);
+    if (sh.s2m_vec)
+        launch_s2m_mma<T>(g, sh, nz, st, x, phi, wstride_T, sums_part, wstride_c64, tb.initch, conj);
+    else
+        hipLaunchKernelGGL((k_s2m<T, PT>), dim3(sh.s2m_nblk, 1, nz), dim3(kS2mThreads), sh.s2m_lds, st, g, x,
+                           g.N, phi, wstride_T, sums_part, wstride_c64, tb.initch, conj);
     hipLaunchKernelGGL(k_sums_reduce, dim3(g.P, 1, nz), dim3(256), 0, st, g, sums_part, sh.s2m_nblk, wstride_c64,
                        sumsec, wstride_c64);
     mark(1);

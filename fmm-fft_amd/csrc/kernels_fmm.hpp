@@ -381,6 +381,121 @@ struct Mfma<double> {
     static __device__ inline int row(int kr, int r) { return kr + 4 * r; }
 };
 
+// ---------------------------------------------------------------------------
+// S2M on the matrix cores, complex64 (leaf size 16, 32 or 64; 2P a
+// multiple of 16 VEC).
+// Same sums as k_s2m: for one box, phi[k][n] = sum_j initch[j][k] B[j][n]
+// with B[j][n] real n of row box*b + j of x (row = P complex = 2P reals,
+// n = 2 sc + re/im), i.e. D = A B with A[k][j] = initch[j][k] zero-padded
+// to 16 MT rows and K = b in KS = b/4 steps.  B comes straight from HBM:
+// in K step s lane l loads VEC consecutive reals of row 4s + l/16, and
+// element e of that vector is column l%16 of N tile e, so N tile e, column
+// c holds real chunk*16VEC + c*VEC + e.  A wave owns one chunk of 16 VEC
+// reals of the row for all its boxes (A fragments in registers for the
+// whole kernel) and adds its reals into FP64 section sums; no LDS until the
+// final per-workgroup reduction of the sums.
+// ---------------------------------------------------------------------------
+template <typename T, int VEC>
+struct VecOf;
+template <> struct VecOf<float, 4> { typedef float4 type; };
+template <> struct VecOf<float, 2> { typedef float2 type; };
+__device__ inline float velem(const float4 &v, int e) { return e == 0 ? v.x : e == 1 ? v.y : e == 2 ? v.z : v.w; }
+__device__ inline float velem(const float2 &v, int e) { return e == 0 ? v.x : v.y; }
+
+template <typename T, int VEC, int MT, int KS>
+__global__ __launch_bounds__(256) void k_s2m_mma(Geom g, const cx<T> *__restrict__ x, int64_t x_stride,
+                                                 T *__restrict__ phi, int64_t tree_stride,
+                                                 c64 *__restrict__ sums_part, int64_t sums_stride,
+                                                 const T *__restrict__ initch, int conj) {
+    using V4 = typename Mfma<T>::v4;
+    using VT = typename VecOf<T, VEC>::type;
+    __shared__ double red[4][16 * VEC];
+    const int P = g.P, terms = g.terms, rows = 2 * g.S;
+    const int nchunk = 2 * P / (16 * VEC), bper = 4 / nchunk;  // chunks of a row, boxes per workgroup step
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, col = lane & 15, kr = lane >> 4;
+    const int chunk = wave % nchunk, bslot = wave / nchunk;
+    const int sig = blockIdx.z;
+    const T *xr = reinterpret_cast<const T *>(x + (int64_t)sig * x_stride);
+    phi += (int64_t)sig * tree_stride + level_off(g, g.n);
+    sums_part += (int64_t)sig * sums_stride + (int64_t)blockIdx.x * P;
+    const int n0 = chunk * 16 * VEC + col * VEC;  // first real of this lane's columns
+    // A fragments: A[16m + col][4s + kr] = initch[4s + kr][16m + col] (zero past terms)
+    T a[MT][KS];
+#pragma unroll
+    for (int m = 0; m < MT; ++m)
+#pragma unroll
+        for (int s = 0; s < KS; ++s) {
+            const int k = 16 * m + col;
+            const T v = initch[(4 * s + kr) * terms + min(k, terms - 1)];
+            a[m][s] = k < terms ? v : T(0);
+        }
+    // the imaginary parts change sign for the conjugated (inverse) input
+    T sgn[VEC];
+#pragma unroll
+    for (int e = 0; e < VEC; ++e) sgn[e] = (conj && ((n0 + e) & 1)) ? T(-1) : T(1);
+    double sum[VEC];
+#pragma unroll
+    for (int e = 0; e < VEC; ++e) sum[e] = 0.0;
+
+    const int64_t rstride = 2 * (int64_t)P;  // reals per row
+    for (int64_t box = (int64_t)blockIdx.x * bper + bslot; box < g.s; box += (int64_t)gridDim.x * bper) {
+        const T *src = xr + (box * KS * 4 + kr) * rstride + n0;
+        VT v[KS];
+#pragma unroll
+        for (int s = 0; s < KS; ++s) v[s] = *reinterpret_cast<const VT *>(src + (int64_t)s * 4 * rstride);
+        V4 acc[MT][VEC];
+#pragma unroll
+        for (int m = 0; m < MT; ++m)
+#pragma unroll
+            for (int e = 0; e < VEC; ++e) acc[m][e] = V4{0, 0, 0, 0};
+#pragma unroll
+        for (int s = 0; s < KS; ++s)
+#pragma unroll
+            for (int e = 0; e < VEC; ++e) {
+                const T bv = velem(v[s], e);
+                sum[e] += (double)bv;
+#pragma unroll
+                for (int m = 0; m < MT; ++m) acc[m][e] = Mfma<T>::mma(a[m][s], bv, acc[m][e]);
+            }
+        // phi[box][k][n - 2] (section 0 has no tree rows)
+        T *dst = phi + box * terms * rows + n0 - 2;
+#pragma unroll
+        for (int m = 0; m < MT; ++m)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int k = 16 * m + Mfma<T>::row(kr, r);
+                if (k < terms) {
+                    T *d = dst + (int64_t)k * rows;
+#pragma unroll
+                    for (int e = 0; e < VEC; e += 2)
+                        if (n0 + e >= 2)
+                            *reinterpret_cast<float2 *>(d + e) =
+                                make_float2(sgn[e] * acc[m][e][r], sgn[e + 1] * acc[m][e + 1][r]);
+                }
+            }
+    }
+    // per-section partial sums of this workgroup, fixed order: lane groups
+    // (kr), then the waves of each chunk
+#pragma unroll
+    for (int e = 0; e < VEC; ++e) {
+        sum[e] += __shfl_xor(sum[e], 16, 64);
+        sum[e] += __shfl_xor(sum[e], 32, 64);
+        if (kr == 0) red[wave][col * VEC + e] = sum[e];
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < P) {
+        double t2[2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int n = 2 * threadIdx.x + h, ch = n / (16 * VEC), w0 = n % (16 * VEC);
+            double acc2 = 0.0;
+            for (int w = ch; w < 4; w += nchunk) acc2 += red[w][w0];
+            t2[h] = acc2;
+        }
+        sums_part[threadIdx.x] = c64{t2[0], conj ? -t2[1] : t2[1]};
+    }
+}
+
 // Waves per SIMD the register allocation of a leaf instance is held to:
 // three for the FP32 leaf-64 instances, which fit 168 registers without
 // spilling (left alone they take ~180 and run two workgroups per CU).
