@@ -411,22 +411,26 @@ void launch_leaf_mma(const Geom &g, const Shapes &sh, unsigned nz, hipStream_t s
     }
 }
 
+// The kernel's discarded stores (section 0, padding terms) go to sumsec,
+// which the section-sum reduction launched after it overwrites.
 template <typename T, int VEC, int MT>
 void s2m_mma_ks(const Geom &g, const Shapes &sh, unsigned nz, hipStream_t st, const cx<T> *x, T *phi,
-                int64_t wstride_T, c64 *sums_part, int64_t wstride_c64, const T *initch, int conj) {
+                int64_t wstride_T, c64 *sums_part, int64_t wstride_c64, const T *initch, c64 *sumsec, int conj) {
     const dim3 grid(sh.s2m_nblk, 1, nz);
+    T *sink = reinterpret_cast<T *>(sumsec);
+    const int64_t sink_stride = wstride_c64 * 16 / (int64_t)sizeof(T);
     switch (g.b) {
         case 16:
             hipLaunchKernelGGL((k_s2m_mma<T, VEC, MT, 4>), grid, dim3(256), 0, st, g, x, g.N, phi, wstride_T,
-                               sums_part, wstride_c64, initch, conj);
+                               sums_part, wstride_c64, initch, sink, sink_stride, conj);
             break;
         case 32:
             hipLaunchKernelGGL((k_s2m_mma<T, VEC, MT, 8>), grid, dim3(256), 0, st, g, x, g.N, phi, wstride_T,
-                               sums_part, wstride_c64, initch, conj);
+                               sums_part, wstride_c64, initch, sink, sink_stride, conj);
             break;
         default:
             hipLaunchKernelGGL((k_s2m_mma<T, VEC, MT, 16>), grid, dim3(256), 0, st, g, x, g.N, phi, wstride_T,
-                               sums_part, wstride_c64, initch, conj);
+                               sums_part, wstride_c64, initch, sink, sink_stride, conj);
             break;
     }
 }
@@ -434,14 +438,15 @@ void s2m_mma_ks(const Geom &g, const Shapes &sh, unsigned nz, hipStream_t st, co
 // matrix-core S2M (complex64): VEC from the plan's shapes, MT = 16-term blocks
 template <typename T>
 void launch_s2m_mma(const Geom &g, const Shapes &sh, unsigned nz, hipStream_t st, const cx<T> *x, T *phi,
-                    int64_t wstride_T, c64 *sums_part, int64_t wstride_c64, const T *initch, int conj) {
+                    int64_t wstride_T, c64 *sums_part, int64_t wstride_c64, const T *initch, c64 *sumsec,
+                    int conj) {
     if constexpr (sizeof(T) == 4) {
         if (sh.s2m_vec == 4) {
-            if (g.terms <= 16) s2m_mma_ks<T, 4, 1>(g, sh, nz, st, x, phi, wstride_T, sums_part, wstride_c64, initch, conj);
-            else s2m_mma_ks<T, 4, 2>(g, sh, nz, st, x, phi, wstride_T, sums_part, wstride_c64, initch, conj);
+            if (g.terms <= 16) s2m_mma_ks<T, 4, 1>(g, sh, nz, st, x, phi, wstride_T, sums_part, wstride_c64, initch, sumsec, conj);
+            else s2m_mma_ks<T, 4, 2>(g, sh, nz, st, x, phi, wstride_T, sums_part, wstride_c64, initch, sumsec, conj);
         } else {
-            if (g.terms <= 16) s2m_mma_ks<T, 2, 1>(g, sh, nz, st, x, phi, wstride_T, sums_part, wstride_c64, initch, conj);
-            else s2m_mma_ks<T, 2, 2>(g, sh, nz, st, x, phi, wstride_T, sums_part, wstride_c64, initch, conj);
+            if (g.terms <= 16) s2m_mma_ks<T, 2, 1>(g, sh, nz, st, x, phi, wstride_T, sums_part, wstride_c64, initch, sumsec, conj);
+            else s2m_mma_ks<T, 2, 2>(g, sh, nz, st, x, phi, wstride_T, sums_part, wstride_c64, initch, sumsec, conj);
         }
     }
 }
@@ -451,7 +456,7 @@ void s2m_tree_pt(const Geom &g, const Shapes &sh, unsigned nz, hipStream_t st, c
                  int64_t wstride_T, c64 *sums_part, int64_t wstride_c64, c64 *sumsec, const TabPtrs<T> &tb,
                  int conj, Mark &mark) {
     if (sh.s2m_vec)
-        launch_s2m_mma<T>(g, sh, nz, st, x, phi, wstride_T, sums_part, wstride_c64, tb.initch, conj);
+        launch_s2m_mma<T>(g, sh, nz, st, x, phi, wstride_T, sums_part, wstride_c64, tb.initch, sumsec, conj);
     else
         hipLaunchKernelGGL((k_s2m<T, PT>), dim3(sh.s2m_nblk, 1, nz), dim3(kS2mThreads), sh.s2m_lds, st, g, x,
                            g.N, phi, wstride_T, sums_part, wstride_c64, tb.initch, conj);

@@ -406,7 +406,8 @@ template <typename T, int VEC, int MT, int KS>
 __global__ __launch_bounds__(256) void k_s2m_mma(Geom g, const cx<T> *__restrict__ x, int64_t x_stride,
                                                  T *__restrict__ phi, int64_t tree_stride,
                                                  c64 *__restrict__ sums_part, int64_t sums_stride,
-                                                 const T *__restrict__ initch, int conj) {
+                                                 const T *__restrict__ initch, T *__restrict__ sink,
+                                                 int64_t sink_stride, int conj) {
     using V4 = typename Mfma<T>::v4;
     using VT = typename VecOf<T, VEC>::type;
     __shared__ double red[4][16 * VEC];
@@ -418,6 +419,7 @@ __global__ __launch_bounds__(256) void k_s2m_mma(Geom g, const cx<T> *__restrict
     const T *xr = reinterpret_cast<const T *>(x + (int64_t)sig * x_stride);
     phi += (int64_t)sig * tree_stride + level_off(g, g.n);
     sums_part += (int64_t)sig * sums_stride + (int64_t)blockIdx.x * P;
+    sink += (int64_t)sig * sink_stride;
     const int n0 = chunk * 16 * VEC + col * VEC;  // first real of this lane's columns
     // A fragments: A[16m + col][4s + kr] = initch[4s + kr][16m + col] (zero past terms)
     T a[MT][KS];
@@ -438,41 +440,65 @@ __global__ __launch_bounds__(256) void k_s2m_mma(Geom g, const cx<T> *__restrict
     for (int e = 0; e < VEC; ++e) sum[e] = 0.0;
 
     const int64_t rstride = 2 * (int64_t)P;  // reals per row
-    for (int64_t box = (int64_t)blockIdx.x * bper + bslot; box < g.s; box += (int64_t)gridDim.x * bper) {
-        const T *src = xr + (box * KS * 4 + kr) * rstride + n0;
-        VT v[KS];
+    // two half boxes in flight: the second half of a box and the first half
+    // of the next box load while the other half computes (the last box
+    // prefetches itself again, keeping every load unconditional)
+    constexpr int H = KS / 2;
+    auto load = [&](VT *v, int64_t bx, int half) __attribute__((always_inline)) {
+        const T *src = xr + (bx * KS * 4 + 4 * H * half + kr) * rstride + n0;
 #pragma unroll
-        for (int s = 0; s < KS; ++s) v[s] = *reinterpret_cast<const VT *>(src + (int64_t)s * 4 * rstride);
+        for (int s = 0; s < H; ++s) v[s] = *reinterpret_cast<const VT *>(src + (int64_t)s * 4 * rstride);
+    };
+    const int64_t step = (int64_t)gridDim.x * bper, last = g.s - 1;
+    int64_t box = (int64_t)blockIdx.x * bper + bslot;
+    VT va[H], vb[H];
+    if (box < g.s) load(va, box, 0);
+    while (box < g.s) {
+        load(vb, box, 1);
         V4 acc[MT][VEC];
 #pragma unroll
         for (int m = 0; m < MT; ++m)
 #pragma unroll
             for (int e = 0; e < VEC; ++e) acc[m][e] = V4{0, 0, 0, 0};
 #pragma unroll
-        for (int s = 0; s < KS; ++s)
+        for (int s = 0; s < H; ++s)
 #pragma unroll
             for (int e = 0; e < VEC; ++e) {
-                const T bv = velem(v[s], e);
+                const T bv = velem(va[s], e);
                 sum[e] += (double)bv;
 #pragma unroll
                 for (int m = 0; m < MT; ++m) acc[m][e] = Mfma<T>::mma(a[m][s], bv, acc[m][e]);
             }
-        // phi[box][k][n - 2] (section 0 has no tree rows)
+        const int64_t nbox = box + step;
+        load(va, nbox < last ? nbox : last, 0);
+#pragma unroll
+        for (int s = 0; s < H; ++s)
+#pragma unroll
+            for (int e = 0; e < VEC; ++e) {
+                const T bv = velem(vb[s], e);
+                sum[e] += (double)bv;
+#pragma unroll
+                for (int m = 0; m < MT; ++m) acc[m][e] = Mfma<T>::mma(a[m][H + s], bv, acc[m][e]);
+            }
+        // phi[box][k][n - 2]; section 0 (no tree rows) and the padding
+        // terms k >= terms go to the sink, so every store is unconditional
+        // (a store under a lane condition becomes a branch, and the counter
+        // wait after a branch covers the loads in flight)
         T *dst = phi + box * terms * rows + n0 - 2;
 #pragma unroll
         for (int m = 0; m < MT; ++m)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int k = 16 * m + Mfma<T>::row(kr, r);
-                if (k < terms) {
-                    T *d = dst + (int64_t)k * rows;
+                T *d = dst + (int64_t)k * rows;
 #pragma unroll
-                    for (int e = 0; e < VEC; e += 2)
-                        if (n0 + e >= 2)
-                            *reinterpret_cast<float2 *>(d + e) =
-                                make_float2(sgn[e] * acc[m][e][r], sgn[e + 1] * acc[m][e + 1][r]);
+                for (int e = 0; e < VEC; e += 2) {
+                    T *de = (k < terms && n0 + e >= 2) ? d + e : sink;
+                    *reinterpret_cast<float2 *>(de) =
+                        make_float2(sgn[e] * acc[m][e][r], sgn[e + 1] * acc[m][e + 1][r]);
                 }
             }
+        box = nbox;
     }
     // per-section partial sums of this workgroup, fixed order: lane groups
     // (kr), then the waves of each chunk
