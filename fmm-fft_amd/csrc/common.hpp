@@ -53,6 +53,21 @@ struct Geom {
     int s, n;       // leaf boxes per section (2^n), tree depth
 };
 
+// Row order of the leaf output (psec) and of the combine output.  Natural
+// (ltw < 0), or column-tiled for a two-pass local FFT of M = M1 x M2
+// (M1 = 2^lm1, M2 = 2^lm2, row mu = n1 M2 + n2): row mu at
+// (n2 / TW) TW M1 + n1 TW + n2 % TW, TW = 2^ltw the column pass's tile
+// width, so that every column-pass tile (TW columns of M1 points) is one
+// contiguous range.  Offsets inside a block fit 32 bits.
+struct RowMap {
+    int ltw, lm1, lm2;
+    __host__ __device__ inline int operator()(int mu) const {
+        if (ltw < 0) return mu;
+        const int n1 = mu >> lm2, n2 = mu & ((1 << lm2) - 1);
+        return ((n2 >> ltw) << (ltw + lm1)) + (n1 << ltw) + (n2 & ((1 << ltw) - 1));
+    }
+};
+
 // L2T K steps (of 4 terms) per row tile of the matrix-core leaf kernel; the
 // evaluation-row table and the kernel's LDS copy of the local expansions are
 // padded to 4 * leaf_kt(terms) terms.

@@ -133,6 +133,7 @@ struct Shapes {
     int s2m_bpb, s2m_nblk;
     size_t s2m_lds;
     int s2m_vec;  // matrix-core S2M: reals per lane load (0: vector-FMA kernel)
+    RowMap rm;    // row order of the leaf output (column-tiled for two FFT passes)
     int leaf_G, leaf_bb;
     size_t leaf_lds;
     bool leaf_mma;  // matrix-core leaf kernel (leaf size a multiple of 16)
@@ -198,6 +199,21 @@ Shapes shapes(const Geom &g, size_t esize) {
     sh.leaf_bb = bb;
     sh.leaf_lds = leaf_lds(g, esize, G, bb);
     sh.leaf_mma = g.b % 16 == 0 && g.b <= 128;  // b is a power of two: b/16 in {1,2,4,8}
+    // two-pass local FFT: the leaf writes psec column-tiled (the combine
+    // keeps the order) in tiles of TW = max(C, 128 bytes) columns, C the
+    // column pass's tile width: the leaf keeps 128-byte output pieces and a
+    // column-pass tile reads C-column pieces at a stride of TW (one 128-byte
+    // line per row for a pair of neighbouring tiles) instead of M2
+    {
+        const FftLayout fl = fft_layout(g);
+        if (fl.two_pass) {
+            const int cols = esize == 4 ? fft_cols<float>((int)fl.M1) : fft_cols<double>((int)fl.M1);
+            const int tw = std::max(cols, (int)(128 / (2 * esize)));
+            sh.rm = RowMap{ilog2(tw), ilog2(fl.M1), ilog2(fl.M2)};
+        } else {
+            sh.rm = RowMap{-1, 0, 0};
+        }
+    }
     if (sh.leaf_mma) {
         // one 64-byte x segment per window row, 8 boxes per workgroup
         // 4 sections per tile (2 when P = 2): one section per wave
@@ -366,7 +382,7 @@ void leaf_mma_inst(const Geom &g, const Shapes &sh, unsigned nz, hipStream_t st,
     nwg64 = std::max<int64_t>(ngroups, nwg64 / ngroups * ngroups);
     const unsigned nwg = (unsigned)nwg64;
     hipLaunchKernelGGL(kern, dim3(nwg, 1, nz), dim3(256), sh.leaf_lds, st, g, x, g.N, psi, wstride_T, psec,
-                       wstride_cx, tb.evfrag, tb.evalm, tb.evalmh, tb.kv, conj);
+                       wstride_cx, tb.evfrag, tb.evalm, tb.evalmh, tb.kv, conj, sh.rm);
 }
 
 template <typename T, int NRT, int G>
@@ -537,7 +553,7 @@ int run(const fmmfft_plan_s *p, int64_t count, const void *d_in, void *d_out, in
     } else {
         hipLaunchKernelGGL(k_leaf<T>, dim3(g.s / sh.leaf_bb, g.P / sh.leaf_G, nz), dim3(256), sh.leaf_lds, st, g,
                            x, g.N, psi, wstride_T, psec, wstride_cx, tb.evalm, tb.evalmh, tb.kv, sh.leaf_G,
-                           sh.leaf_bb, conj);
+                           sh.leaf_bb, conj, sh.rm);
     }
     mark(4);
     launch_combine<T>(g, nz, st, psec, wstride_cx, sumsec, wstride_c64, gfac, wP, out, g.N);
@@ -549,7 +565,7 @@ int run(const fmmfft_plan_s *p, int64_t count, const void *d_in, void *d_out, in
     if (!f.two_pass) {
         FftPass fp{};
         fp.F = 1;
-        fp.in_fs = g.M; fp.in_es = 1;
+        fp.in_fs = g.M; fp.in_es = 1; fp.in_bs = 0; fp.in_ltw = 30;
         fp.out_fs = g.M; fp.out_ks = 1;
         fp.conj_out = conj;
         fp.logR = f.logR;
@@ -558,7 +574,11 @@ int run(const fmmfft_plan_s *p, int64_t count, const void *d_in, void *d_out, in
     } else {
         FftPass a{};
         a.F = f.M2;
-        a.in_fs = 1; a.in_es = f.M2;
+        // column-tiled input (the leaf's row map, tiles of TW >= C columns):
+        // column f0 + j of tile q = f0 / TW at q TW M1 + f0 % TW + j, element
+        // e at e TW
+        a.in_fs = 1; a.in_es = int64_t(1) << sh.rm.ltw;
+        a.in_bs = f.M1 << sh.rm.ltw; a.in_ltw = sh.rm.ltw;
         a.out_fs = 1; a.out_ks = f.M2;
         a.twiddle = 1;
         a.logR = f.logR;
@@ -566,7 +586,7 @@ int run(const fmmfft_plan_s *p, int64_t count, const void *d_in, void *d_out, in
         mark(6);
         FftPass bq{};
         bq.F = f.M1;
-        bq.in_fs = f.M2; bq.in_es = 1;
+        bq.in_fs = f.M2; bq.in_es = 1; bq.in_bs = 0; bq.in_ltw = 30;
         bq.out_fs = 1; bq.out_ks = f.M1;
         bq.conj_out = conj;
         bq.logR = f.logR;

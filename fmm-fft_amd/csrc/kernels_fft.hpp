@@ -18,7 +18,11 @@ namespace fmmfft {
 
 struct FftPass {
     int64_t F;             // sub-FFTs per length-M block
-    int64_t in_fs, in_es;  // element e of sub-FFT f at k2*M + f*in_fs + e*in_es
+    int64_t in_fs, in_es;  // element e of sub-FFT f at k2*M + f*in_fs + e*in_es;
+    int64_t in_bs;         // column-tiled input: f = f0 + j (f0 the tile's first,
+    int in_ltw;            // a multiple of its C), f0 = q 2^in_ltw + r, at
+                           // k2*M + q*in_bs + (r + j)*in_fs + e*in_es
+                           // (in_ltw = 30: q = 0, the plain form)
     int64_t out_fs, out_ks;
     int twiddle;           // 1: multiply output k of sub-FFT f by w_M^{f k}
     int conj_out;          // conjugate the output (inverse transform)
@@ -180,11 +184,13 @@ constexpr int tile_stride() {
 
 // Sub-FFTs per workgroup: 8192 (complex64) or 4096 (complex128) elements,
 // at most 32; about 70 KB of LDS.
-template <typename T, int L>
-constexpr int fft_cols() {
-    constexpr int elems = sizeof(T) == 4 ? 8192 : 4096;
+template <typename T>
+constexpr int fft_cols(int L) {
+    const int elems = sizeof(T) == 4 ? 8192 : 4096;
     return elems / L > 32 ? 32 : elems / L;
 }
+template <typename T, int L>
+constexpr int fft_cols() { return fft_cols<T>(L); }
 
 // Workgroup index remap: the round-robin dispatch puts consecutive workgroup
 // ids on different XCDs (each with its own L2); remap so that each XCD runs a
@@ -228,6 +234,7 @@ __global__ __launch_bounds__(kFftThreads) void k_fft(Geom g, FftPass fp, const c
     const int xo = (int)xcd_remap(blockIdx.x, nwg);
     // when F == 1 the C sub-FFTs of a tile are C consecutive blocks
     const int fstep_in = F == 1 ? M : (int)fp.in_fs;
+    const int fblk_in = (int)fp.in_bs, in_ltw = fp.in_ltw;
     const int fstep_out = F == 1 ? M : (int)fp.out_fs;
     const int in_es = (int)fp.in_es, out_ks = (int)fp.out_ks;
     // single pass: e and k run along rows; pass 1: both strided (columns of
@@ -246,7 +253,8 @@ __global__ __launch_bounds__(kFftThreads) void k_fft(Geom g, FftPass fp, const c
         for (int u = 0; u < PER; ++u) {
             const int i = threadIdx.x + u * NT;
             const int f = erow ? i / L : i % C, e = erow ? i % L : i / C;
-            const cx<T> x = in[base + (f0 + min(f, cl)) * fstep_in + e * in_es];
+            const int fo = (f0 >> in_ltw) * fblk_in + ((f0 & ((1 << in_ltw) - 1)) + min(f, cl)) * fstep_in;
+            const cx<T> x = in[base + fo + e * in_es];
             vr[u] = x.re;
             vi[u] = x.im;
         }

@@ -199,7 +199,7 @@ __global__ __launch_bounds__(256) void k_leaf(Geom g, const cx<T> *__restrict__ 
                                               const T *__restrict__ psi, int64_t tree_stride,
                                               cx<T> *__restrict__ psec, int64_t psec_stride,
                                               const T *__restrict__ evalm, const T *__restrict__ evalmh,
-                                              const T *__restrict__ kv, int G, int bb, int conj) {
+                                              const T *__restrict__ kv, int G, int bb, int conj, RowMap rm) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int P = g.P, b = g.b, terms = g.terms, S = g.S;
     const int64_t M = g.M;
@@ -251,7 +251,8 @@ __global__ __launch_bounds__(256) void k_leaf(Geom g, const cx<T> *__restrict__ 
         const int j0 = 4 * jq;
         const int64_t mu0 = (int64_t)(i0 + bl) * b + j0;
         if (sc == 0) {
-            for (int r = 0; r < 4; ++r) psec[mu0 + r] = xw[((bl + 2) * b + j0 + r) * G + gg];
+            // four consecutive rows stay consecutive (mu0 % 4 = 0, TW >= 4)
+            for (int r = 0; r < 4; ++r) psec[rm((int)mu0) + r] = xw[((bl + 2) * b + j0 + r) * G + gg];
             continue;
         }
         const T *kr = kvs + gg * kvl;
@@ -306,7 +307,7 @@ __global__ __launch_bounds__(256) void k_leaf(Geom g, const cx<T> *__restrict__ 
                 aim[0] = him;
             }
         }
-        for (int r = 0; r < 4; ++r) psec[(int64_t)sc * M + mu0 + r] = cx<T>{are[r], aim[r]};
+        for (int r = 0; r < 4; ++r) psec[(int64_t)sc * M + rm((int)mu0) + r] = cx<T>{are[r], aim[r]};
     }
 }
 
@@ -538,7 +539,8 @@ void k_leaf_mma(Geom g, const cx<T> *__restrict__ x, int64_t x_stride,
                                                   const T *__restrict__ psi, int64_t tree_stride,
                                                   cx<T> *__restrict__ psec, int64_t psec_stride,
                                                   const T *__restrict__ evfrag, const T *__restrict__ evalm,
-                                                  const T *__restrict__ evalmh, const T *__restrict__ kv, int conj) {
+                                                  const T *__restrict__ evalmh, const T *__restrict__ kv, int conj,
+                                                  RowMap rm) {
     using V4 = typename Mfma<T>::v4;
     constexpr int BB = 8 * CT;       // boxes per tile: CT column tiles of 8 boxes x re/im
     constexpr int b = 16 * NRT;      // leaf size
@@ -679,8 +681,8 @@ void k_leaf_mma(Geom g, const cx<T> *__restrict__ x, int64_t x_stride,
 
         if (gg < G) {
             if (sc == 0) {
-                cx<T> *dst = psec + (int64_t)i0 * b;
-                for (int t = lane; t < BB * b; t += 64) dst[t] = xw[(2 * b + t) * G + boxpad((2 * b + t) / b)];
+                for (int t = lane; t < BB * b; t += 64)
+                    psec[rm(i0 * b + t)] = xw[(2 * b + t) * G + boxpad((2 * b + t) / b)];
             } else {
                 const T *kr_ = kvs + gg * 4 * b;
                 V4 acc[NRT][CT];
@@ -788,7 +790,8 @@ void k_leaf_mma(Geom g, const cx<T> *__restrict__ x, int64_t x_stride,
                 for (int ct = 0; ct < CT; ++ct)
 #pragma unroll
                 for (int rt = 0; rt < NRT; ++rt) {
-                    cx<T> *dst = psec + (int64_t)sc * M + (int64_t)(i0 + bx + 8 * ct) * b;
+                    cx<T> *dst = psec + (int64_t)sc * M;
+                    const int mu = (i0 + bx + 8 * ct) * b + 16 * rt;
                     const T a0 = acc[rt][ct][0], a1 = acc[rt][ct][1], a2 = acc[rt][ct][2], a3 = acc[rt][ct][3];
                     const T keep0 = bsel(c, a2, a0), keep1 = bsel(c, a3, a1);
                     const T recv0 = swap_pair(bsel(c, a0, a2));
@@ -796,12 +799,13 @@ void k_leaf_mma(Geom g, const cx<T> *__restrict__ x, int64_t x_stride,
                     const cx<T> o0{bsel(c, recv0, keep0), bsel(c, keep0, recv0)};
                     const cx<T> o1{bsel(c, recv1, keep1), bsel(c, keep1, recv1)};
                     if constexpr (sizeof(T) == 4) {
-                        // rows 4 kr + 2c, 4 kr + 2c + 1: one 16-byte store
-                        *reinterpret_cast<float4 *>(dst + 16 * rt + Mfma<T>::row(kr, 2 * c)) =
+                        // rows 4 kr + 2c, 4 kr + 2c + 1: one 16-byte store (an
+                        // even pair stays a pair in the row map, TW >= 4)
+                        *reinterpret_cast<float4 *>(dst + rm(mu + Mfma<T>::row(kr, 2 * c))) =
                             make_float4(o0.re, o0.im, o1.re, o1.im);
                     } else {
-                        dst[16 * rt + Mfma<T>::row(kr, 2 * c)] = o0;
-                        dst[16 * rt + Mfma<T>::row(kr, 2 * c + 1)] = o1;
+                        dst[rm(mu + Mfma<T>::row(kr, 2 * c))] = o0;
+                        dst[rm(mu + Mfma<T>::row(kr, 2 * c + 1))] = o1;
                     }
                 }
             }
@@ -944,6 +948,8 @@ __global__ __launch_bounds__(256) void k_tree_mma(Geom g, int l, const T *__rest
 // reference's MPI_Alltoall).  Tiles of 32 rows; reads and writes are 32
 // contiguous elements per section / block.  Persistent workgroups prefetch
 // the next tile into registers while transforming the current one.
+// Rows are independent, so the kernel is blind to the row order: psec and y
+// share whatever layout the leaf kernel wrote (natural or column-tiled).
 // ---------------------------------------------------------------------------
 constexpr int kCombineThreads = 256;
 
