@@ -209,7 +209,8 @@ Shapes shapes(const Geom &g, size_t esize) {
         if (fl.two_pass) {
             const int cols = esize == 4 ? fft_cols<float>((int)fl.M1) : fft_cols<double>((int)fl.M1);
             const int tw = std::max(cols, (int)(128 / (2 * esize)));
-            sh.rm = RowMap{ilog2(tw), ilog2(fl.M1), ilog2(fl.M2)};
+            // (tw <= 32 <= M2 whenever two passes run; kept as a guard)
+            sh.rm = tw <= fl.M2 ? RowMap{ilog2(tw), ilog2(fl.M1), ilog2(fl.M2)} : RowMap{-1, 0, 0};
         } else {
             sh.rm = RowMap{-1, 0, 0};
         }
@@ -574,11 +575,15 @@ int run(const fmmfft_plan_s *p, int64_t count, const void *d_in, void *d_out, in
     } else {
         FftPass a{};
         a.F = f.M2;
-        // column-tiled input (the leaf's row map, tiles of TW >= C columns):
-        // column f0 + j of tile q = f0 / TW at q TW M1 + f0 % TW + j, element
-        // e at e TW
-        a.in_fs = 1; a.in_es = int64_t(1) << sh.rm.ltw;
-        a.in_bs = f.M1 << sh.rm.ltw; a.in_ltw = sh.rm.ltw;
+        if (sh.rm.ltw >= 0) {
+            // column-tiled input (the leaf's row map, tiles of TW >= C
+            // columns): column f0 + j of tile q = f0 / TW at
+            // q TW M1 + f0 % TW + j, element e at e TW
+            a.in_fs = 1; a.in_es = int64_t(1) << sh.rm.ltw;
+            a.in_bs = f.M1 << sh.rm.ltw; a.in_ltw = sh.rm.ltw;
+        } else {
+            a.in_fs = 1; a.in_es = f.M2; a.in_bs = 0; a.in_ltw = 30;
+        }
         a.out_fs = 1; a.out_ks = f.M2;
         a.twiddle = 1;
         a.logR = f.logR;
