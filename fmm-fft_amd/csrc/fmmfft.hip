@@ -174,7 +174,8 @@ Shapes shapes(const Geom &g, size_t esize) {
     sh.s2m_lds = (size_t)g.b * padded_terms(g.terms) * esize;
     // matrix-core S2M for complex64, leaf 16..64: a row of x (2P reals)
     // splits into chunks of 16 VEC reals, VEC = 4 (P >= 32) or 2 (P = 16);
-    // one chunk per wave, 4 / chunks boxes per workgroup step, at most 768
+    // one chunk per wave, 4 / chunks sibling box pairs per workgroup step
+    // (the kernel also computes their parents, level n-1), at most 768
     // workgroups (three per CU: the leaf-64 instances hold 3 to 5 per CU).
     // complex128 keeps the vector-FMA kernel, which already streams x at
     // copy speed (the FP64 VALU rate equals the FP32 one here).
@@ -186,8 +187,8 @@ Shapes shapes(const Geom &g, size_t esize) {
         }
         sh.s2m_vec = vec;
         if (vec) {
-            const int nchunk = 2 * g.P / (16 * vec), bper = 4 / nchunk;
-            sh.s2m_nblk = std::min((g.s + bper - 1) / bper, 768);
+            const int nchunk = 2 * g.P / (16 * vec), pper = 4 / nchunk;
+            sh.s2m_nblk = std::min((g.s / 2 + pper - 1) / pper, 768);
         }
     }
 
@@ -432,22 +433,23 @@ void launch_leaf_mma(const Geom &g, const Shapes &sh, unsigned nz, hipStream_t s
 // which the section-sum reduction launched after it overwrites.
 template <typename T, int VEC, int MT>
 void s2m_mma_ks(const Geom &g, const Shapes &sh, unsigned nz, hipStream_t st, const cx<T> *x, T *phi,
-                int64_t wstride_T, c64 *sums_part, int64_t wstride_c64, const T *initch, c64 *sumsec, int conj) {
+                int64_t wstride_T, c64 *sums_part, int64_t wstride_c64, const T *initch, const T *opfrag,
+                c64 *sumsec, int conj) {
     const dim3 grid(sh.s2m_nblk, 1, nz);
     T *sink = reinterpret_cast<T *>(sumsec);
     const int64_t sink_stride = wstride_c64 * 16 / (int64_t)sizeof(T);
     switch (g.b) {
         case 16:
             hipLaunchKernelGGL((k_s2m_mma<T, VEC, MT, 4>), grid, dim3(256), 0, st, g, x, g.N, phi, wstride_T,
-                               sums_part, wstride_c64, initch, sink, sink_stride, conj);
+                               sums_part, wstride_c64, initch, opfrag, sink, sink_stride, conj);
             break;
         case 32:
             hipLaunchKernelGGL((k_s2m_mma<T, VEC, MT, 8>), grid, dim3(256), 0, st, g, x, g.N, phi, wstride_T,
-                               sums_part, wstride_c64, initch, sink, sink_stride, conj);
+                               sums_part, wstride_c64, initch, opfrag, sink, sink_stride, conj);
             break;
         default:
             hipLaunchKernelGGL((k_s2m_mma<T, VEC, MT, 16>), grid, dim3(256), 0, st, g, x, g.N, phi, wstride_T,
-                               sums_part, wstride_c64, initch, sink, sink_stride, conj);
+                               sums_part, wstride_c64, initch, opfrag, sink, sink_stride, conj);
             break;
     }
 }
@@ -455,15 +457,15 @@ void s2m_mma_ks(const Geom &g, const Shapes &sh, unsigned nz, hipStream_t st, co
 // matrix-core S2M (complex64): VEC from the plan's shapes, MT = 16-term blocks
 template <typename T>
 void launch_s2m_mma(const Geom &g, const Shapes &sh, unsigned nz, hipStream_t st, const cx<T> *x, T *phi,
-                    int64_t wstride_T, c64 *sums_part, int64_t wstride_c64, const T *initch, c64 *sumsec,
-                    int conj) {
+                    int64_t wstride_T, c64 *sums_part, int64_t wstride_c64, const T *initch, const T *opfrag,
+                    c64 *sumsec, int conj) {
     if constexpr (sizeof(T) == 4) {
         if (sh.s2m_vec == 4) {
-            if (g.terms <= 16) s2m_mma_ks<T, 4, 1>(g, sh, nz, st, x, phi, wstride_T, sums_part, wstride_c64, initch, sumsec, conj);
-            else s2m_mma_ks<T, 4, 2>(g, sh, nz, st, x, phi, wstride_T, sums_part, wstride_c64, initch, sumsec, conj);
+            if (g.terms <= 16) s2m_mma_ks<T, 4, 1>(g, sh, nz, st, x, phi, wstride_T, sums_part, wstride_c64, initch, opfrag, sumsec, conj);
+            else s2m_mma_ks<T, 4, 2>(g, sh, nz, st, x, phi, wstride_T, sums_part, wstride_c64, initch, opfrag, sumsec, conj);
         } else {
-            if (g.terms <= 16) s2m_mma_ks<T, 2, 1>(g, sh, nz, st, x, phi, wstride_T, sums_part, wstride_c64, initch, sumsec, conj);
-            else s2m_mma_ks<T, 2, 2>(g, sh, nz, st, x, phi, wstride_T, sums_part, wstride_c64, initch, sumsec, conj);
+            if (g.terms <= 16) s2m_mma_ks<T, 2, 1>(g, sh, nz, st, x, phi, wstride_T, sums_part, wstride_c64, initch, opfrag, sumsec, conj);
+            else s2m_mma_ks<T, 2, 2>(g, sh, nz, st, x, phi, wstride_T, sums_part, wstride_c64, initch, opfrag, sumsec, conj);
         }
     }
 }
@@ -473,7 +475,8 @@ void s2m_tree_pt(const Geom &g, const Shapes &sh, unsigned nz, hipStream_t st, c
                  int64_t wstride_T, c64 *sums_part, int64_t wstride_c64, c64 *sumsec, const TabPtrs<T> &tb,
                  int conj, Mark &mark) {
     if (sh.s2m_vec)
-        launch_s2m_mma<T>(g, sh, nz, st, x, phi, wstride_T, sums_part, wstride_c64, tb.initch, sumsec, conj);
+        launch_s2m_mma<T>(g, sh, nz, st, x, phi, wstride_T, sums_part, wstride_c64, tb.initch, tb.opfrag, sumsec,
+                          conj);
     else
         hipLaunchKernelGGL((k_s2m<T, PT>), dim3(sh.s2m_nblk, 1, nz), dim3(kS2mThreads), sh.s2m_lds, st, g, x,
                            g.N, phi, wstride_T, sums_part, wstride_c64, tb.initch, conj);
@@ -489,7 +492,8 @@ void s2m_tree_pt(const Geom &g, const Shapes &sh, unsigned nz, hipStream_t st, c
         const int64_t waves = (int64_t(1) << l) * ((nrb + rb - 1) / rb);
         return dim3((unsigned)((waves + 3) / 4), 1, nz);
     };
-    for (int l = g.n - 1; l >= 2; --l)
+    // the matrix-core S2M has already formed level n-1
+    for (int l = g.n - 1 - (sh.s2m_vec ? 1 : 0); l >= 2; --l)
         hipLaunchKernelGGL((k_tree_mma<T, TP, 0>), grid(l, 0), dim3(256), 0, st, g, l, (const T *)phi, phi,
                            (const T *)psi, psi, wstride_T, tb.opfrag);
     mark(2);

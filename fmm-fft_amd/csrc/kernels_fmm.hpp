@@ -394,7 +394,9 @@ struct Mfma<double> {
 // c holds real chunk*16VEC + c*VEC + e.  A wave owns one chunk of 16 VEC
 // reals of the row for all its boxes (A fragments in registers for the
 // whole kernel) and adds its reals into FP64 section sums; no LDS until the
-// final per-workgroup reduction of the sums.
+// final per-workgroup reduction of the sums.  Boxes come in sibling pairs
+// and the first upward level (M2M to level n-1, src/fmmfft.c:476-515) runs
+// on the pair's accumulators before they leave registers.
 // ---------------------------------------------------------------------------
 template <typename T, int VEC>
 struct VecOf;
@@ -407,18 +409,20 @@ template <typename T, int VEC, int MT, int KS>
 __global__ __launch_bounds__(256) void k_s2m_mma(Geom g, const cx<T> *__restrict__ x, int64_t x_stride,
                                                  T *__restrict__ phi, int64_t tree_stride,
                                                  c64 *__restrict__ sums_part, int64_t sums_stride,
-                                                 const T *__restrict__ initch, T *__restrict__ sink,
-                                                 int64_t sink_stride, int conj) {
+                                                 const T *__restrict__ initch, const T *__restrict__ opfrag,
+                                                 T *__restrict__ sink, int64_t sink_stride, int conj) {
     using V4 = typename Mfma<T>::v4;
     using VT = typename VecOf<T, VEC>::type;
+    constexpr int TP = 16 * MT, KT = 4 * MT, FR = TP * TP;  // the tree's padded terms and K steps
     __shared__ double red[4][16 * VEC];
     const int P = g.P, terms = g.terms, rows = 2 * g.S;
-    const int nchunk = 2 * P / (16 * VEC), bper = 4 / nchunk;  // chunks of a row, boxes per workgroup step
+    const int nchunk = 2 * P / (16 * VEC), pper = 4 / nchunk;  // chunks of a row, box pairs per workgroup step
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, col = lane & 15, kr = lane >> 4;
-    const int chunk = wave % nchunk, bslot = wave / nchunk;
+    const int chunk = wave % nchunk, pslot = wave / nchunk;
     const int sig = blockIdx.z;
     const T *xr = reinterpret_cast<const T *>(x + (int64_t)sig * x_stride);
-    phi += (int64_t)sig * tree_stride + level_off(g, g.n);
+    T *phi_n = phi + (int64_t)sig * tree_stride + level_off(g, g.n);      // leaves
+    T *phi_p = phi + (int64_t)sig * tree_stride + level_off(g, g.n - 1);  // their parents
     sums_part += (int64_t)sig * sums_stride + (int64_t)blockIdx.x * P;
     sink += (int64_t)sig * sink_stride;
     const int n0 = chunk * 16 * VEC + col * VEC;  // first real of this lane's columns
@@ -432,6 +436,25 @@ __global__ __launch_bounds__(256) void k_s2m_mma(Geom g, const cx<T> *__restrict
             const T v = initch[(4 * s + kr) * terms + min(k, terms - 1)];
             a[m][s] = k < terms ? v : T(0);
         }
+    // M2M to level n-1 on the accumulators in place: in K step s lane (kr, col)
+    // holds term 16 (s/4) + 4 kr + s%4 of its columns (the D layout), so the
+    // operator fragments are read with that K order: am[c][ko][s] =
+    // A_c[16 ko + col][16 (s/4) + 4 kr + s%4] from the fragment-ordered table
+    // (element A[i][j] at ((j/4) KO + ko) 64 + i%16 + 16 (j%4))
+    T am[2][MT][KT];
+    {
+        const T *o = opfrag + (size_t)(g.n - 3) * NOPS * FR;
+        const T *op[2] = {o + OP_SHFTFN * FR, o + OP_SHFTFP * FR};
+#pragma unroll
+        for (int c2 = 0; c2 < 2; ++c2)
+#pragma unroll
+            for (int ko = 0; ko < MT; ++ko)
+#pragma unroll
+                for (int s = 0; s < KT; ++s) {
+                    const int j = 16 * (s / 4) + 4 * kr + (s % 4);
+                    am[c2][ko][s] = op[c2][((j / 4) * MT + ko) * 64 + col + 16 * (j % 4)];
+                }
+    }
     // the imaginary parts change sign for the conjugated (inverse) input
     T sgn[VEC];
 #pragma unroll
@@ -441,51 +464,33 @@ __global__ __launch_bounds__(256) void k_s2m_mma(Geom g, const cx<T> *__restrict
     for (int e = 0; e < VEC; ++e) sum[e] = 0.0;
 
     const int64_t rstride = 2 * (int64_t)P;  // reals per row
-    // two half boxes in flight: the second half of a box and the first half
-    // of the next box load while the other half computes (the last box
-    // prefetches itself again, keeping every load unconditional)
+    // a wave takes sibling boxes (2q, 2q+1) and then their parent q; two
+    // half boxes of loads in flight: the next half loads while the current
+    // one computes (the last pair prefetches itself again, keeping every
+    // load unconditional)
     constexpr int H = KS / 2;
     auto load = [&](VT *v, int64_t bx, int half) __attribute__((always_inline)) {
         const T *src = xr + (bx * KS * 4 + 4 * H * half + kr) * rstride + n0;
 #pragma unroll
         for (int s = 0; s < H; ++s) v[s] = *reinterpret_cast<const VT *>(src + (int64_t)s * 4 * rstride);
     };
-    const int64_t step = (int64_t)gridDim.x * bper, last = g.s - 1;
-    int64_t box = (int64_t)blockIdx.x * bper + bslot;
-    VT va[H], vb[H];
-    if (box < g.s) load(va, box, 0);
-    while (box < g.s) {
-        load(vb, box, 1);
-        V4 acc[MT][VEC];
-#pragma unroll
-        for (int m = 0; m < MT; ++m)
-#pragma unroll
-            for (int e = 0; e < VEC; ++e) acc[m][e] = V4{0, 0, 0, 0};
+    auto mma_half = [&](V4 (&acc)[MT][VEC], const VT *v, int half) __attribute__((always_inline)) {
 #pragma unroll
         for (int s = 0; s < H; ++s)
 #pragma unroll
             for (int e = 0; e < VEC; ++e) {
-                const T bv = velem(va[s], e);
+                const T bv = velem(v[s], e);
                 sum[e] += (double)bv;
 #pragma unroll
-                for (int m = 0; m < MT; ++m) acc[m][e] = Mfma<T>::mma(a[m][s], bv, acc[m][e]);
+                for (int m = 0; m < MT; ++m) acc[m][e] = Mfma<T>::mma(a[m][H * half + s], bv, acc[m][e]);
             }
-        const int64_t nbox = box + step;
-        load(va, nbox < last ? nbox : last, 0);
-#pragma unroll
-        for (int s = 0; s < H; ++s)
-#pragma unroll
-            for (int e = 0; e < VEC; ++e) {
-                const T bv = velem(vb[s], e);
-                sum[e] += (double)bv;
-#pragma unroll
-                for (int m = 0; m < MT; ++m) acc[m][e] = Mfma<T>::mma(a[m][H + s], bv, acc[m][e]);
-            }
-        // phi[box][k][n - 2]; section 0 (no tree rows) and the padding
-        // terms k >= terms go to the sink, so every store is unconditional
-        // (a store under a lane condition becomes a branch, and the counter
-        // wait after a branch covers the loads in flight)
-        T *dst = phi + box * terms * rows + n0 - 2;
+    };
+    // node [k][n - 2] of a level; section 0 (no tree rows) and the padding
+    // terms k >= terms go to the sink, so every store is unconditional (a
+    // store under a lane condition becomes a branch, and the counter wait
+    // after a branch covers the loads in flight)
+    auto store = [&](T *node, const V4 (&acc)[MT][VEC]) __attribute__((always_inline)) {
+        T *dst = node + n0 - 2;
 #pragma unroll
         for (int m = 0; m < MT; ++m)
 #pragma unroll
@@ -499,7 +504,48 @@ __global__ __launch_bounds__(256) void k_s2m_mma(Geom g, const cx<T> *__restrict
                         make_float2(sgn[e] * acc[m][e][r], sgn[e + 1] * acc[m][e + 1][r]);
                 }
             }
-        box = nbox;
+    };
+    auto zero = [&](V4 (&acc)[MT][VEC]) __attribute__((always_inline)) {
+#pragma unroll
+        for (int m = 0; m < MT; ++m)
+#pragma unroll
+            for (int e = 0; e < VEC; ++e) acc[m][e] = V4{0, 0, 0, 0};
+    };
+    const int64_t npairs = g.s / 2, step = (int64_t)gridDim.x * pper, lastp = npairs - 1;
+    const int64_t node_len = (int64_t)terms * rows;
+    int64_t q = (int64_t)blockIdx.x * pper + pslot;
+    VT va[H], vb[H];
+    if (q < npairs) load(va, 2 * q, 0);
+    while (q < npairs) {
+        V4 acc[2][MT][VEC];
+        load(vb, 2 * q, 1);
+        zero(acc[0]);
+        mma_half(acc[0], va, 0);
+        load(va, 2 * q + 1, 0);
+        mma_half(acc[0], vb, 1);
+        store(phi_n + 2 * q * node_len, acc[0]);
+        load(vb, 2 * q + 1, 1);
+        zero(acc[1]);
+        mma_half(acc[1], va, 0);
+        const int64_t nq = q + step;
+        load(va, 2 * (nq < lastp ? nq : lastp), 0);
+        mma_half(acc[1], vb, 1);
+        store(phi_n + (2 * q + 1) * node_len, acc[1]);
+        // parent q = shftfn . child 2q + shftfp . child 2q+1
+        V4 up[MT][VEC];
+        zero(up);
+#pragma unroll
+        for (int c2 = 0; c2 < 2; ++c2)
+#pragma unroll
+            for (int s = 0; s < KT; ++s)
+#pragma unroll
+                for (int e = 0; e < VEC; ++e) {
+                    const T bv = acc[c2][s / 4][e][s % 4];
+#pragma unroll
+                    for (int ko = 0; ko < MT; ++ko) up[ko][e] = Mfma<T>::mma(am[c2][ko][s], bv, up[ko][e]);
+                }
+        store(phi_p + q * node_len, up);
+        q = nq;
     }
     // per-section partial sums of this workgroup, fixed order: lane groups
     // (kr), then the waves of each chunk
