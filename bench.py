@@ -265,6 +265,52 @@ def measured_traffic(stage):
     return None
 
 
+def path_roofline(stage_ms, esize=8, peak_tflops=PEAK_FP32_TFLOPS):
+    """Roofline time of the whole path at this workload: the sum over stages
+    of max(algorithmic flops / matrix peak, algorithmic bytes / HBM peak),
+    and its ratio to the measured stage times.  Algorithmic counts (a
+    complex x real MAC = 4 flops; a tree level l holds 2^l nodes x S
+    sections x p complex terms): S2M reads x, writes tree levels n and n-1
+    (the first M2M runs in the same kernel); the upward pass reads levels
+    n-1..3 and writes n-2..2; the downward pass reads phi_l and psi_(l-1)
+    and writes psi_l for l = 2..n (one parent L2L and three M2L flips per
+    node, one flip at the top); the leaf reads x and psi_n and writes the
+    section sums; the combine and each FFT pass read and write N points."""
+    M = N // P_BLOCKS
+    S = P_BLOCKS - 1
+    s = M // LEAF
+    n = int(math.log2(s))
+    p = TERMS
+    lvl = lambda l: (1 << l) * S * p * esize  # bytes of one tree level
+    m1 = 1 << ((int(math.log2(M)) + 1) // 2)
+    m2 = M // m1
+    nb = N * esize
+    count = {
+        "s2m+sums": (4.0 * S * M * p + 8.0 * p * p * S * (s // 2), nb + lvl(n) + lvl(n - 1)),
+        "upward": (8.0 * p * p * S * ((1 << (n - 1)) - 4),
+                   sum(lvl(l + 1) + lvl(l) for l in range(2, n - 1))),
+        "downward": (16.0 * p * p * S * sum(1 << l for l in range(3, n + 1)) + 4.0 * p * p * S * 4,
+                     sum(2 * lvl(l) + lvl(l - 1) for l in range(3, n + 1)) + 2 * lvl(2)),
+        "leaf": (4.0 * S * M * (3 * LEAF + p), nb + lvl(n) + nb),
+        "combine": (5.0 * N * math.log2(P_BLOCKS), 2 * nb),
+        "fft1": (5.0 * N * math.log2(m1), 2 * nb),
+        "fft2": (5.0 * N * math.log2(m2), 2 * nb),
+    }
+    phases, t_roof, t_meas = {}, 0.0, 0.0
+    for st, (fl, by) in count.items():
+        if st not in stage_ms:
+            continue
+        tf = fl / (peak_tflops * 1e12) * 1e3
+        tb = by / (PEAK_HBM_GBS * 1e9) * 1e3
+        t = max(tf, tb)
+        t_roof += t
+        t_meas += stage_ms[st]
+        phases[st] = {"bound": "mfma" if tf >= tb else "hbm", "roof_ms": round(t, 4),
+                      "measured_ms": round(stage_ms[st], 4), "frac": round(t / stage_ms[st], 3)}
+    return {"t_roofline_ms": round(t_roof, 4), "t_measured_ms": round(t_meas, 4),
+            "frac": round(t_roof / t_meas, 3), "phases": phases}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -360,6 +406,7 @@ def main():
         "latency_ms": {"p50": round(pct(lat32, 50), 4), "p99": round(pct(lat32, 99), 4)},
         "stage_ms": {k: round(v, 4) for k, v in per_stage.items()},
         "roofline": roof,
+        "path_roofline": path_roofline(per_stage),
     }
     line.update(extra)
     if not args.no_serve:
