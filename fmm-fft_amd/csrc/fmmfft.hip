@@ -147,10 +147,12 @@ size_t leaf_lds(const Geom &g, size_t esize, int G, int bb) {
            (size_t)(bb + 1) * G * 2 * g.terms * esize;
 }
 
-// Column tiles (of 8 boxes) per leaf tile: leaf 32 and 16 take 2 and 4 so
-// that a tile holds as many MFMAs as one of leaf 64 (at most s / 8).
+// Column tiles (of 8 boxes) per leaf tile: leaf 16 takes 4 so that a tile
+// holds as many MFMAs as one of leaf 64 (at most s / 8); leaf 32 measured
+// faster with one column tile at four waves per SIMD than with two at two
+// (batched 256 x 2^20 leaf stage 2.32 -> 2.17 ms).
 int leaf_ct(const Geom &g) {
-    const int ct = g.b >= 64 ? 1 : g.b == 32 ? 2 : 4;
+    const int ct = g.b >= 32 ? 1 : 4;
     return std::min(ct, g.s / 8);
 }
 

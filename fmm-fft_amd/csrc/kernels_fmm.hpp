@@ -571,10 +571,12 @@ __global__ __launch_bounds__(256) void k_s2m_mma(Geom g, const cx<T> *__restrict
 
 // Waves per SIMD the register allocation of a leaf instance is held to:
 // three for the FP32 leaf-64 instances, which fit 168 registers without
-// spilling (left alone they take ~180 and run two workgroups per CU).
+// spilling (left alone they take ~180 and run two workgroups per CU), four
+// for FP32 leaf 32 (128 registers), two for FP64 leaf 64.
 template <typename T, int NRT, int G, int KT, int CT>
 constexpr int leaf_waves() {
     if (sizeof(T) == 4 && G == 4 && NRT == 4 && CT == 1) return 3;
+    if (sizeof(T) == 4 && G == 4 && NRT == 2 && CT == 1) return 4;
     if (sizeof(T) == 8 && G == 4 && NRT == 4 && CT == 1 && KT == 4) return 2;
     return 1;
 }
