@@ -489,21 +489,31 @@ void s2m_tree_pt(const Geom &g, const Shapes &sh, unsigned nz, hipStream_t st, c
     // (node, block of 16 rows), four waves per workgroup
     constexpr int TP = PT <= 16 ? 16 : 32;
     const int64_t nrb = (2 * (int64_t)g.S + 15) / 16;
+    // nodes of at most two row blocks (P <= 16): packed form, several
+    // nodes per wave
+    const bool pack = nrb <= 2;
     auto grid = [&](int l, int mode) {
-        const int rb = tree_mma_rb<T, TP>(mode);
-        const int64_t waves = (int64_t(1) << l) * ((nrb + rb - 1) / rb);
+        const int64_t nn = int64_t(1) << l;
+        int64_t waves;
+        if (pack) {
+            const int64_t npw = tree_mma_rb_packed<T, TP>(mode) / nrb;
+            waves = 2 * ((nn + 2 * npw - 1) / (2 * npw));
+        } else {
+            const int rb = tree_mma_rb<T, TP>(mode);
+            waves = nn * ((nrb + rb - 1) / rb);
+        }
         return dim3((unsigned)((waves + 3) / 4), 1, nz);
+    };
+    auto level = [&](auto kern, auto pkern, int l, int mode) {
+        hipLaunchKernelGGL(pack ? pkern : kern, grid(l, mode), dim3(256), 0, st, g, l, (const T *)phi, phi,
+                           (const T *)psi, psi, wstride_T, tb.opfrag);
     };
     // the matrix-core S2M has already formed level n-1
     for (int l = g.n - 1 - (sh.s2m_vec ? 1 : 0); l >= 2; --l)
-        hipLaunchKernelGGL((k_tree_mma<T, TP, 0>), grid(l, 0), dim3(256), 0, st, g, l, (const T *)phi, phi,
-                           (const T *)psi, psi, wstride_T, tb.opfrag);
+        level(k_tree_mma<T, TP, 0, false>, k_tree_mma<T, TP, 0, true>, l, 0);
     mark(2);
-    hipLaunchKernelGGL((k_tree_mma<T, TP, 2>), grid(2, 2), dim3(256), 0, st, g, 2, (const T *)phi, phi,
-                       (const T *)psi, psi, wstride_T, tb.opfrag);
-    for (int l = 3; l <= g.n; ++l)
-        hipLaunchKernelGGL((k_tree_mma<T, TP, 1>), grid(l, 1), dim3(256), 0, st, g, l, (const T *)phi, phi,
-                           (const T *)psi, psi, wstride_T, tb.opfrag);
+    level(k_tree_mma<T, TP, 2, false>, k_tree_mma<T, TP, 2, true>, 2, 2);
+    for (int l = 3; l <= g.n; ++l) level(k_tree_mma<T, TP, 1, false>, k_tree_mma<T, TP, 1, true>, l, 1);
 }
 
 template <typename T, typename Mark>

@@ -884,6 +884,12 @@ void k_leaf_mma(Geom g, const cx<T> *__restrict__ x, int64_t x_stride,
 // c +- 2, 3 are shared in one XCD's L2.  TP = terms padded to 16 or 32:
 // TP/4 K steps, TP/16 output term tiles.
 // ---------------------------------------------------------------------------
+// row blocks per wave of the packed form (nodes of at most two row blocks)
+template <typename T, int TP>
+__host__ __device__ constexpr int tree_mma_rb_packed(int mode) {
+    return mode == 2 ? 4 : (sizeof(T) == 4 && TP == 16) ? 8 : 4;
+}
+
 template <typename T, int TP>
 __host__ __device__ constexpr int tree_mma_rb(int mode) {
     // row blocks per wave: the B fragments of one wave stay within ~64 VGPRs;
@@ -893,7 +899,11 @@ __host__ __device__ constexpr int tree_mma_rb(int mode) {
                        : (64 * 4 / ((mode == 0 ? 2 : 4) * (TP / 4) * (int)sizeof(T))) >= 2 ? 2 : 1;
 }
 
-template <typename T, int TP, int MODE>
+// Packed form (PACK, for levels whose nodes have at most two row blocks,
+// i.e. P <= 16): a wave takes RB / nrb nodes of one parity (c = 2 RB/nrb
+// (w/2) + w%2 + 2k) with all their row blocks, so small nodes do not cost a
+// wave (and a memory round trip) each.
+template <typename T, int TP, int MODE, bool PACK = false>
 __global__ __launch_bounds__(256) void k_tree_mma(Geom g, int l, const T *__restrict__ phi_in,
                                                   T *__restrict__ phi_out, const T *__restrict__ psi_in,
                                                   T *__restrict__ psi_out, int64_t tree_stride,
@@ -901,48 +911,65 @@ __global__ __launch_bounds__(256) void k_tree_mma(Geom g, int l, const T *__rest
     using V4 = typename Mfma<T>::v4;
     constexpr int KS = TP / 4, KO = TP / 16;
     constexpr int NT = MODE == 0 ? 2 : MODE == 1 ? 4 : 1;  // sources per node
-    constexpr int RB = tree_mma_rb<T, TP>(MODE);
+    constexpr int RB = PACK ? tree_mma_rb_packed<T, TP>(MODE) : tree_mma_rb<T, TP>(MODE);
     constexpr int FR = TP * TP;                             // one operator in fragment order
     const int terms = g.terms, rows = 2 * g.S;
     const int node_len = rows * terms;
     const int nn = 1 << l, nrb = (rows + 15) >> 4, nw = (nrb + RB - 1) / RB;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     const int col = lane & 15, kr = lane >> 4;
     const int wid = 4 * (int)xcd_remap(blockIdx.x, gridDim.x) + wave;
-    const int c = wid / nw, rb0 = (wid - c * nw) * RB;
+    const int npw = PACK ? RB / nrb : 1;  // nodes per wave
+    // first node of the wave and its parity (the same for all its nodes)
+    const int c = PACK ? 2 * npw * (wid >> 1) + (wid & 1) : wid / nw;
+    const int rb0 = PACK ? 0 : (wid - c * nw) * RB;
     if (c >= nn) return;
     const int64_t ts = (int64_t)blockIdx.z * tree_stride;
-    const int odd = c & 1;
+    const int odd = c & 1, m = nn - 1;
+    // node and row block of item j
+    auto item = [&](int j, int &cj, int &rbj) __attribute__((always_inline)) {
+        if (PACK) {
+            cj = c + 2 * (j / nrb);
+            rbj = j - (j / nrb) * nrb;
+        } else {
+            cj = c;
+            rbj = rb0 + j;
+        }
+    };
 
     const T *mats[NT];
-    const T *src[NT];
-    T *dst;
     if constexpr (MODE == 0) {
         const T *o = opfrag + (size_t)(l - 2) * NOPS * FR;
         mats[0] = o + OP_SHFTFN * FR;
         mats[1] = o + OP_SHFTFP * FR;
-        const T *ch = phi_in + ts + level_off(g, l + 1);
-        src[0] = ch + (int64_t)(2 * c) * node_len;
-        src[1] = ch + (int64_t)(2 * c + 1) * node_len;
-        dst = phi_out + ts + level_off(g, l) + (int64_t)c * node_len;
     } else if constexpr (MODE == 1) {
         const T *o = opfrag + (size_t)(l - 3) * NOPS * FR;
         mats[0] = o + (odd ? OP_SHFTLN : OP_SHFTLP) * FR;
         mats[1] = o + (odd ? OP_FLIP3P : OP_FLIP2P) * FR;
         mats[2] = o + (odd ? OP_FLIP2P : OP_FLIP2N) * FR;
         mats[3] = o + (odd ? OP_FLIP2N : OP_FLIP3N) * FR;
-        const T *ph = phi_in + ts + level_off(g, l);
-        const int m = nn - 1;
-        src[0] = psi_in + ts + level_off(g, l - 1) + (int64_t)(c >> 1) * node_len;
-        src[1] = ph + (int64_t)((c + (odd ? -3 : -2)) & m) * node_len;
-        src[2] = ph + (int64_t)((c + (odd ? -2 : 2)) & m) * node_len;
-        src[3] = ph + (int64_t)((c + (odd ? 2 : 3)) & m) * node_len;
-        dst = psi_out + ts + level_off(g, l) + (int64_t)c * node_len;
     } else {
         mats[0] = opfrag + (size_t)(g.n - 2) * NOPS * FR;  // topflp
-        src[0] = phi_in + ts + level_off(g, 2) + (int64_t)((c + 2) & 3) * node_len;
-        dst = psi_out + ts + level_off(g, 2) + (int64_t)c * node_len;
     }
+    // source nodes of node cn (clamped to the level: a packed wave's nodes
+    // past the end re-read valid ones and store nothing)
+    auto sources = [&](int cn, const T *(&src)[NT]) __attribute__((always_inline)) {
+        cn = min(cn, m);
+        if constexpr (MODE == 0) {
+            const T *ch = phi_in + ts + level_off(g, l + 1);
+            src[0] = ch + (int64_t)(2 * cn) * node_len;
+            src[1] = ch + (int64_t)(2 * cn + 1) * node_len;
+        } else if constexpr (MODE == 1) {
+            const T *ph = phi_in + ts + level_off(g, l);
+            src[0] = psi_in + ts + level_off(g, l - 1) + (int64_t)(cn >> 1) * node_len;
+            src[1] = ph + (int64_t)((cn + (odd ? -3 : -2)) & m) * node_len;
+            src[2] = ph + (int64_t)((cn + (odd ? -2 : 2)) & m) * node_len;
+            src[3] = ph + (int64_t)((cn + (odd ? 2 : 3)) & m) * node_len;
+        } else {
+            src[0] = phi_in + ts + level_off(g, 2) + (int64_t)((cn + 2) & 3) * node_len;
+        }
+    };
+    T *lvl_out = (MODE == 0 ? phi_out : psi_out) + ts + level_off(g, l);
     // all loads of the wave at once, unconditional (a load under a lane
     // condition becomes a branch that waits for it): operator fragments,
     // then the RB row blocks' source fragments at clamped indices.  No
@@ -957,7 +984,11 @@ __global__ __launch_bounds__(256) void k_tree_mma(Geom g, int l, const T *__rest
             for (int ko = 0; ko < KO; ++ko) a[t][ko][s] = mats[t][(s * KO + ko) * 64 + lane];
 #pragma unroll
     for (int j = 0; j < RB; ++j) {
-        const int rc = min(16 * (rb0 + j) + col, rows - 1);
+        int cj, rbj;
+        item(j, cj, rbj);
+        const T *src[NT];
+        sources(cj, src);
+        const int rc = min(16 * rbj + col, rows - 1);
 #pragma unroll
         for (int t = 0; t < NT; ++t)
 #pragma unroll
@@ -965,8 +996,11 @@ __global__ __launch_bounds__(256) void k_tree_mma(Geom g, int l, const T *__rest
     }
 #pragma unroll
     for (int j = 0; j < RB; ++j) {
-        const int row = 16 * (rb0 + j) + col;
-        if (16 * (rb0 + j) >= rows) break;  // wave-uniform
+        int cj, rbj;
+        item(j, cj, rbj);
+        const int row = 16 * rbj + col;
+        if (16 * rbj >= rows || cj >= nn) break;  // wave-uniform; items run in node order
+        T *dst = lvl_out + (int64_t)cj * node_len;
         V4 acc[KO];
 #pragma unroll
         for (int ko = 0; ko < KO; ++ko) acc[ko] = V4{0, 0, 0, 0};
