@@ -320,10 +320,15 @@ void fft_len(const Geom &g, const FftPass &fp, unsigned nz, hipStream_t st, cons
         // FP64 pass 2 measures faster as one tile per workgroup (no
         // persistence): its strided 64-byte output pieces gain from many
         // workgroups in flight more than from the register prefetch
-        const void *k = (const void *)k_fft<T, L, FFT_PASS2>;
-        const dim3 grid(sizeof(T) == 8 ? (unsigned)ntiles : resident_grid(k, kFftThreads, 0, ntiles), 1, nz);
-        hipLaunchKernelGGL((k_fft<T, L, FFT_PASS2>), grid, block, 0, st, g, fp, in, out, in_stride, out_stride, tw,
-                           tlo, thi);
+        if constexpr (sizeof(T) == 8) {
+            hipLaunchKernelGGL((k_fft<T, L, FFT_PASS2, C, false>), dim3((unsigned)ntiles, 1, nz), block, 0, st, g,
+                               fp, in, out, in_stride, out_stride, tw, tlo, thi);
+        } else {
+            const void *k = (const void *)k_fft<T, L, FFT_PASS2>;
+            const dim3 grid(resident_grid(k, kFftThreads, 0, ntiles), 1, nz);
+            hipLaunchKernelGGL((k_fft<T, L, FFT_PASS2>), grid, block, 0, st, g, fp, in, out, in_stride, out_stride,
+                               tw, tlo, thi);
+        }
     } else {
         const void *k = (const void *)k_fft<T, L, FFT_PASS1>;
         const dim3 grid(resident_grid(k, kFftThreads, 0, ntiles), 1, nz);

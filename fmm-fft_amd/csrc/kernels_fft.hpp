@@ -213,7 +213,8 @@ constexpr int kFftThreads = 512;
 // Access forms of a pass: element rows / output rows contiguous or strided.
 enum FftForm { FFT_SINGLE = 0, FFT_PASS1 = 1, FFT_PASS2 = 2 };
 
-template <typename T, int L, int FORM, int C = fft_cols<T, L>()>
+// PERSIST = false: one tile per workgroup (grid = tiles), no prefetch.
+template <typename T, int L, int FORM, int C = fft_cols<T, L>(), bool PERSIST = true>
 __global__ __launch_bounds__(kFftThreads) void k_fft(Geom g, FftPass fp, const cx<T> *__restrict__ in,
                                              cx<T> *__restrict__ out, int64_t in_stride, int64_t out_stride,
                                              const cx<T> *__restrict__ twL, const c64 *__restrict__ tlo,
@@ -271,21 +272,27 @@ __global__ __launch_bounds__(kFftThreads) void k_fft(Geom g, FftPass fp, const c
             a[f * ls + lpad(e)] = cx<T>{vr[u], vi[u]};
         }
         __syncthreads();
-        const int tn = t + nwg;
-        if (tn < ntiles) prefetch(tn);
-        stockham<T, L, C, 1, NT>(a, ls, twL);
-
         const int fg0 = t * C;
         const int cv = min(C, nf - fg0);
         const int k2 = fg0 / F, f0 = fg0 - k2 * F, base = k2 * M;
+        // pass-1 twiddle factors of this tile, loaded before the next tile's
+        // prefetch so that their use waits for them alone (unconditional:
+        // the indices are valid for every form)
+        // pass 1 (krow false): lane's f = tid % C fixed, k = tid / C + u * (NT / C)
+        const int fa = f0 + (int)threadIdx.x % C;
+        const int e0 = (int)(((int64_t)fa * (threadIdx.x / C)) & (M - 1));
+        const int es = (int)(((int64_t)fa * (NT / C)) & (M - 1));
+        const c64 th0 = thi[e0 >> fp.logR], tl0 = tlo[e0 & Rm], ths = thi[es >> fp.logR], tls = tlo[es & Rm];
+        // the last iteration prefetches a valid tile it never uses (an
+        // unconditional prefetch keeps the counter waits exact)
+        const int tn = t + nwg;
+        if constexpr (PERSIST) prefetch(min(tn, ntiles - 1));
+        stockham<T, L, C, 1, NT>(a, ls, twL);
+
         c64 w{1.0, 0.0}, ws{1.0, 0.0};
         if (fp.twiddle) {
-            // pass 1 (krow false): lane's f = tid % C fixed, k = tid / C + u * (NT / C)
-            const int fa = f0 + (int)threadIdx.x % C;
-            const int e0 = (int)(((int64_t)fa * (threadIdx.x / C)) & (M - 1));
-            const int es = (int)(((int64_t)fa * (NT / C)) & (M - 1));
-            w = cmul(thi[e0 >> fp.logR], tlo[e0 & Rm]);
-            ws = cmul(thi[es >> fp.logR], tlo[es & Rm]);
+            w = cmul(th0, tl0);
+            ws = cmul(ths, tls);
         }
 #pragma unroll
         for (int u = 0; u < PER; ++u) {
@@ -293,8 +300,14 @@ __global__ __launch_bounds__(kFftThreads) void k_fft(Geom g, FftPass fp, const c
             const int f = krow ? i / L : i % C, k = krow ? i % L : i / C;
             cx<T> val = a[f * ls + lpad(k)];
             if (fp.twiddle) {
-                const c64 r = cmul(c64{(double)val.re, (double)val.im}, w);
-                val = cx<T>{(T)r.re, (T)r.im};
+                // FP64 recurrence, applied in the element's precision
+                // (complex64: an FP32 product with the rounded factor)
+                if constexpr (sizeof(T) == 4) {
+                    val = cmulT(val, cx<T>{(T)w.re, (T)w.im});
+                } else {
+                    const c64 r = cmul(c64{(double)val.re, (double)val.im}, w);
+                    val = cx<T>{(T)r.re, (T)r.im};
+                }
                 w = cmul(w, ws);
             }
             if (fp.conj_out) val.im = -val.im;
