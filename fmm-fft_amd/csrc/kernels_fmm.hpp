@@ -994,30 +994,69 @@ __global__ __launch_bounds__(256) void k_tree_mma(Geom g, int l, const T *__rest
 #pragma unroll
             for (int s = 0; s < KS; ++s) bv[j][t][s] = src[t][min(4 * s + kr, terms - 1) * rows + rc];
     }
+    if constexpr (!PACK) {
+        // the RB items' accumulator chains interleaved (independent MFMAs
+        // back to back; a single chain waits on each result); items past
+        // the level compute on clamped data and store nothing
+        V4 acc[RB][KO];
 #pragma unroll
-    for (int j = 0; j < RB; ++j) {
-        int cj, rbj;
-        item(j, cj, rbj);
-        const int row = 16 * rbj + col;
-        if (16 * rbj >= rows || cj >= nn) break;  // wave-uniform; items run in node order
-        T *dst = lvl_out + (int64_t)cj * node_len;
-        V4 acc[KO];
+        for (int j = 0; j < RB; ++j)
 #pragma unroll
-        for (int ko = 0; ko < KO; ++ko) acc[ko] = V4{0, 0, 0, 0};
+            for (int ko = 0; ko < KO; ++ko) acc[j][ko] = V4{0, 0, 0, 0};
 #pragma unroll
         for (int t = 0; t < NT; ++t)
 #pragma unroll
             for (int s = 0; s < KS; ++s)
 #pragma unroll
-                for (int ko = 0; ko < KO; ++ko) acc[ko] = Mfma<T>::mma(a[t][ko][s], bv[j][t][s], acc[ko]);
-        if (row < rows) {
+                for (int j = 0; j < RB; ++j)
 #pragma unroll
-            for (int ko = 0; ko < KO; ++ko)
+                    for (int ko = 0; ko < KO; ++ko)
+                        acc[j][ko] = Mfma<T>::mma(a[t][ko][s], bv[j][t][s], acc[j][ko]);
 #pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const int k = 16 * ko + Mfma<T>::row(kr, r);
-                    if (k < terms) dst[k * rows + row] = acc[ko][r];
-                }
+        for (int j = 0; j < RB; ++j) {
+            int cj, rbj;
+            item(j, cj, rbj);
+            const int row = 16 * rbj + col;
+            if (16 * rbj >= rows || cj >= nn) break;  // wave-uniform; items run in order
+            T *dst = lvl_out + (int64_t)cj * node_len;
+            if (row < rows) {
+#pragma unroll
+                for (int ko = 0; ko < KO; ++ko)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int k = 16 * ko + Mfma<T>::row(kr, r);
+                        if (k < terms) dst[k * rows + row] = acc[j][ko][r];
+                    }
+            }
+        }
+    } else {
+        // item by item (interleaving all of them made the compiler issue
+        // the loads one at a time to save registers)
+#pragma unroll
+        for (int j = 0; j < RB; ++j) {
+            int cj, rbj;
+            item(j, cj, rbj);
+            const int row = 16 * rbj + col;
+            if (16 * rbj >= rows || cj >= nn) break;  // wave-uniform; items run in node order
+            T *dst = lvl_out + (int64_t)cj * node_len;
+            V4 acc[KO];
+#pragma unroll
+            for (int ko = 0; ko < KO; ++ko) acc[ko] = V4{0, 0, 0, 0};
+#pragma unroll
+            for (int t = 0; t < NT; ++t)
+#pragma unroll
+                for (int s = 0; s < KS; ++s)
+#pragma unroll
+                    for (int ko = 0; ko < KO; ++ko) acc[ko] = Mfma<T>::mma(a[t][ko][s], bv[j][t][s], acc[ko]);
+            if (row < rows) {
+#pragma unroll
+                for (int ko = 0; ko < KO; ++ko)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int k = 16 * ko + Mfma<T>::row(kr, r);
+                        if (k < terms) dst[k * rows + row] = acc[ko][r];
+                    }
+            }
         }
     }
 }
