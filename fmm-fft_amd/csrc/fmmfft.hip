@@ -305,36 +305,38 @@ unsigned resident_grid(const void *kern, int threads, size_t lds, int64_t ntiles
     return (unsigned)std::min<int64_t>(ntiles, (int64_t)std::max(cus, 8) * per_cu);
 }
 
-template <typename T, int L>
-void fft_len(const Geom &g, const FftPass &fp, unsigned nz, hipStream_t st, const cx<T> *in, cx<T> *out,
-             int64_t in_stride, int64_t out_stride, const cx<T> *tw, const c64 *tlo, const c64 *thi) {
+// one pass of sub-FFTs of length L: persistent when the resident grid is
+// smaller than the tile count (register prefetch of the next tile), else one
+// tile per workgroup (no prefetch)
+template <typename T, int L, int FORM>
+void fft_form(const Geom &g, const FftPass &fp, unsigned nz, hipStream_t st, const cx<T> *in, cx<T> *out,
+              int64_t in_stride, int64_t out_stride, const cx<T> *tw, const c64 *tlo, const c64 *thi,
+              bool force_single) {
     constexpr int C = fft_cols<T, L>();
     const int64_t ntiles = (fp.F * g.P + C - 1) / C;
     const dim3 block(kFftThreads);
-    if (fp.in_es == 1 && fp.out_ks == 1) {
-        const void *k = (const void *)k_fft<T, L, FFT_SINGLE>;
-        const dim3 grid(resident_grid(k, kFftThreads, 0, ntiles), 1, nz);
-        hipLaunchKernelGGL((k_fft<T, L, FFT_SINGLE>), grid, block, 0, st, g, fp, in, out, in_stride, out_stride, tw,
-                           tlo, thi);
-    } else if (fp.in_es == 1) {
+    const unsigned res =
+        force_single ? (unsigned)ntiles : resident_grid((const void *)k_fft<T, L, FORM, C, true>, kFftThreads, 0, ntiles);
+    if ((int64_t)res < ntiles)
+        hipLaunchKernelGGL((k_fft<T, L, FORM, C, true>), dim3(res, 1, nz), block, 0, st, g, fp, in, out, in_stride,
+                           out_stride, tw, tlo, thi);
+    else
+        hipLaunchKernelGGL((k_fft<T, L, FORM, C, false>), dim3((unsigned)ntiles, 1, nz), block, 0, st, g, fp, in,
+                           out, in_stride, out_stride, tw, tlo, thi);
+}
+
+template <typename T, int L>
+void fft_len(const Geom &g, const FftPass &fp, unsigned nz, hipStream_t st, const cx<T> *in, cx<T> *out,
+             int64_t in_stride, int64_t out_stride, const cx<T> *tw, const c64 *tlo, const c64 *thi) {
+    if (fp.in_es == 1 && fp.out_ks == 1)
+        fft_form<T, L, FFT_SINGLE>(g, fp, nz, st, in, out, in_stride, out_stride, tw, tlo, thi, false);
+    else if (fp.in_es == 1)
         // FP64 pass 2 measures faster as one tile per workgroup (no
         // persistence): its strided 64-byte output pieces gain from many
         // workgroups in flight more than from the register prefetch
-        if constexpr (sizeof(T) == 8) {
-            hipLaunchKernelGGL((k_fft<T, L, FFT_PASS2, C, false>), dim3((unsigned)ntiles, 1, nz), block, 0, st, g,
-                               fp, in, out, in_stride, out_stride, tw, tlo, thi);
-        } else {
-            const void *k = (const void *)k_fft<T, L, FFT_PASS2>;
-            const dim3 grid(resident_grid(k, kFftThreads, 0, ntiles), 1, nz);
-            hipLaunchKernelGGL((k_fft<T, L, FFT_PASS2>), grid, block, 0, st, g, fp, in, out, in_stride, out_stride,
-                               tw, tlo, thi);
-        }
-    } else {
-        const void *k = (const void *)k_fft<T, L, FFT_PASS1>;
-        const dim3 grid(resident_grid(k, kFftThreads, 0, ntiles), 1, nz);
-        hipLaunchKernelGGL((k_fft<T, L, FFT_PASS1>), grid, block, 0, st, g, fp, in, out, in_stride, out_stride, tw,
-                           tlo, thi);
-    }
+        fft_form<T, L, FFT_PASS2>(g, fp, nz, st, in, out, in_stride, out_stride, tw, tlo, thi, sizeof(T) == 8);
+    else
+        fft_form<T, L, FFT_PASS1>(g, fp, nz, st, in, out, in_stride, out_stride, tw, tlo, thi, false);
 }
 
 template <typename T>

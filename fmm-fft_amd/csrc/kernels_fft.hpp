@@ -276,21 +276,27 @@ __global__ __launch_bounds__(kFftThreads) void k_fft(Geom g, FftPass fp, const c
         const int cv = min(C, nf - fg0);
         const int k2 = fg0 / F, f0 = fg0 - k2 * F, base = k2 * M;
         // pass-1 twiddle factors of this tile, loaded before the next tile's
-        // prefetch so that their use waits for them alone (unconditional:
-        // the indices are valid for every form)
-        // pass 1 (krow false): lane's f = tid % C fixed, k = tid / C + u * (NT / C)
-        const int fa = f0 + (int)threadIdx.x % C;
-        const int e0 = (int)(((int64_t)fa * (threadIdx.x / C)) & (M - 1));
-        const int es = (int)(((int64_t)fa * (NT / C)) & (M - 1));
-        const c64 th0 = thi[e0 >> fp.logR], tl0 = tlo[e0 & Rm], ths = thi[es >> fp.logR], tls = tlo[es & Rm];
-        // the last iteration prefetches a valid tile it never uses (an
-        // unconditional prefetch keeps the counter waits exact)
+        // prefetch so that their use waits for them alone
+        // (lane's f = tid % C fixed, k = tid / C + u * (NT / C))
+        c64 th0{1.0, 0.0}, tl0{1.0, 0.0}, ths{1.0, 0.0}, tls{1.0, 0.0};
+        if constexpr (FORM == FFT_PASS1) {
+            const int fa = f0 + (int)threadIdx.x % C;
+            const int e0 = (int)(((int64_t)fa * (threadIdx.x / C)) & (M - 1));
+            const int es = (int)(((int64_t)fa * (NT / C)) & (M - 1));
+            th0 = thi[e0 >> fp.logR];
+            tl0 = tlo[e0 & Rm];
+            ths = thi[es >> fp.logR];
+            tls = tlo[es & Rm];
+        }
+        // past the last tile the prefetch re-reads the current one (still in
+        // L2, and the prefetch stays unconditional, which keeps the
+        // counter waits exact)
         const int tn = t + nwg;
-        if constexpr (PERSIST) prefetch(min(tn, ntiles - 1));
+        if constexpr (PERSIST) prefetch(tn < ntiles ? tn : t);
         stockham<T, L, C, 1, NT>(a, ls, twL);
 
         c64 w{1.0, 0.0}, ws{1.0, 0.0};
-        if (fp.twiddle) {
+        if (FORM == FFT_PASS1 && fp.twiddle) {
             w = cmul(th0, tl0);
             ws = cmul(ths, tls);
         }
@@ -299,7 +305,7 @@ __global__ __launch_bounds__(kFftThreads) void k_fft(Geom g, FftPass fp, const c
             const int i = threadIdx.x + u * NT;
             const int f = krow ? i / L : i % C, k = krow ? i % L : i / C;
             cx<T> val = a[f * ls + lpad(k)];
-            if (fp.twiddle) {
+            if (FORM == FFT_PASS1 && fp.twiddle) {
                 // FP64 recurrence, applied in the element's precision
                 // (complex64: an FP32 product with the rounded factor)
                 if constexpr (sizeof(T) == 4) {

@@ -511,7 +511,7 @@ __global__ __launch_bounds__(256) void k_s2m_mma(Geom g, const cx<T> *__restrict
 #pragma unroll
             for (int e = 0; e < VEC; ++e) acc[m][e] = V4{0, 0, 0, 0};
     };
-    const int64_t npairs = g.s / 2, step = (int64_t)gridDim.x * pper, lastp = npairs - 1;
+    const int64_t npairs = g.s / 2, step = (int64_t)gridDim.x * pper;
     const int64_t node_len = (int64_t)terms * rows;
     int64_t q = (int64_t)blockIdx.x * pper + pslot;
     VT va[H], vb[H];
@@ -528,7 +528,7 @@ __global__ __launch_bounds__(256) void k_s2m_mma(Geom g, const cx<T> *__restrict
         zero(acc[1]);
         mma_half(acc[1], va, 0);
         const int64_t nq = q + step;
-        load(va, 2 * (nq < lastp ? nq : lastp), 0);
+        load(va, 2 * (nq < npairs ? nq : q), 0);  // past the end: this pair again (in L2)
         mma_half(acc[1], vb, 1);
         store(phi_n + (2 * q + 1) * node_len, acc[1]);
         // parent q = shftfn . child 2q + shftfp . child 2q+1
@@ -722,10 +722,10 @@ void k_leaf_mma(Geom g, const cx<T> *__restrict__ x, int64_t x_stride,
         commit();
         __syncthreads();
         const int i0 = (lid / ngroups) * BB;
-        // the last iteration prefetches a valid tile it never uses (an
-        // unconditional prefetch keeps the counter waits exact)
+        // past the last tile the prefetch re-reads the current one (still in
+        // L2; an unconditional prefetch keeps the counter waits exact)
         const int nlid = (++k) * nwg + xo;
-        prefetch(min(nlid, ntiles - 1));
+        prefetch(nlid < ntiles ? nlid : lid);
 
         if (gg < G) {
             if (sc == 0) {
