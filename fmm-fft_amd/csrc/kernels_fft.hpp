@@ -277,9 +277,13 @@ __global__ __launch_bounds__(kFftThreads) void k_fft(Geom g, FftPass fp, const c
         const int k2 = fg0 / F, f0 = fg0 - k2 * F, base = k2 * M;
         // pass-1 twiddle factors of this tile, loaded before the next tile's
         // prefetch so that their use waits for them alone
-        // (lane's f = tid % C fixed, k = tid / C + u * (NT / C))
+        // (lane's f = tid % C fixed, k = tid / C + u * (NT / C)).  complex128
+        // keeps the loads and the run-time twiddle test in every form:
+        // compiling them out of pass 2 changed the schedule of its output
+        // loop and measured 0.54 -> 0.61 ms at M = 2^20.
+        constexpr bool TWF = FORM == FFT_PASS1 || sizeof(T) == 8;
         c64 th0{1.0, 0.0}, tl0{1.0, 0.0}, ths{1.0, 0.0}, tls{1.0, 0.0};
-        if constexpr (FORM == FFT_PASS1) {
+        if constexpr (TWF) {
             const int fa = f0 + (int)threadIdx.x % C;
             const int e0 = (int)(((int64_t)fa * (threadIdx.x / C)) & (M - 1));
             const int es = (int)(((int64_t)fa * (NT / C)) & (M - 1));
@@ -296,7 +300,7 @@ __global__ __launch_bounds__(kFftThreads) void k_fft(Geom g, FftPass fp, const c
         stockham<T, L, C, 1, NT>(a, ls, twL);
 
         c64 w{1.0, 0.0}, ws{1.0, 0.0};
-        if (FORM == FFT_PASS1 && fp.twiddle) {
+        if (TWF && fp.twiddle) {
             w = cmul(th0, tl0);
             ws = cmul(ths, tls);
         }
@@ -305,7 +309,7 @@ __global__ __launch_bounds__(kFftThreads) void k_fft(Geom g, FftPass fp, const c
             const int i = threadIdx.x + u * NT;
             const int f = krow ? i / L : i % C, k = krow ? i % L : i / C;
             cx<T> val = a[f * ls + lpad(k)];
-            if (FORM == FFT_PASS1 && fp.twiddle) {
+            if (TWF && fp.twiddle) {
                 // FP64 recurrence, applied in the element's precision
                 // (complex64: an FP32 product with the rounded factor)
                 if constexpr (sizeof(T) == 4) {
