@@ -386,14 +386,15 @@ void leaf_mma_inst(const Geom &g, const Shapes &sh, unsigned nz, hipStream_t st,
         (void)hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh.leaf_lds);
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 256, sh.leaf_lds) != hipSuccess || per_cu < 1)
         per_cu = 1;
-    // a multiple of the section groups: each workgroup keeps one group
+    // persistent over the tiles of all nz signals; a multiple of the section
+    // groups: each workgroup keeps one group
     const int64_t ngroups = g.P / G;
-    const int64_t ntiles = ngroups * (g.s / (8 * CT));
-    int64_t nwg64 = std::min<int64_t>(ntiles, (int64_t)std::max(cus, 8) * per_cu);
+    const int64_t ntot = ngroups * (g.s / (8 * CT)) * (int64_t)nz;
+    int64_t nwg64 = std::min<int64_t>(ntot, (int64_t)std::max(cus, 8) * per_cu);
     nwg64 = std::max<int64_t>(ngroups, nwg64 / ngroups * ngroups);
     const unsigned nwg = (unsigned)nwg64;
-    hipLaunchKernelGGL(kern, dim3(nwg, 1, nz), dim3(256), sh.leaf_lds, st, g, x, g.N, psi, wstride_T, psec,
-                       wstride_cx, tb.evfrag, tb.evalm, tb.evalmh, tb.kv, conj, sh.rm);
+    hipLaunchKernelGGL(kern, dim3(nwg, 1, 1), dim3(256), sh.leaf_lds, st, g, x, g.N, psi, wstride_T, psec,
+                       wstride_cx, tb.evfrag, tb.evalm, tb.evalmh, tb.kv, conj, sh.rm, (int)nz);
 }
 
 template <typename T, int NRT, int G>

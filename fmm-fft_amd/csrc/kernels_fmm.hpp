@@ -588,7 +588,7 @@ void k_leaf_mma(Geom g, const cx<T> *__restrict__ x, int64_t x_stride,
                                                   cx<T> *__restrict__ psec, int64_t psec_stride,
                                                   const T *__restrict__ evfrag, const T *__restrict__ evalm,
                                                   const T *__restrict__ evalmh, const T *__restrict__ kv, int conj,
-                                                  RowMap rm) {
+                                                  RowMap rm, int nsig) {
     using V4 = typename Mfma<T>::v4;
     constexpr int BB = 8 * CT;       // boxes per tile: CT column tiles of 8 boxes x re/im
     constexpr int b = 16 * NRT;      // leaf size
@@ -622,18 +622,17 @@ void k_leaf_mma(Geom g, const cx<T> *__restrict__ x, int64_t x_stride,
     T *kvs = reinterpret_cast<T *>(xw + XN + XPAD);        // [G][4b]
     T *lsx = kvs + KVN;                                    // [BB+1][TP][G][2] (+2 per box)
     T *evs = lsx + (BB + 1) * LSTR;                        // [G][2][TP]
-    const int sig = blockIdx.z;
-    x += (int64_t)sig * x_stride;
-    psi += (int64_t)sig * tree_stride + level_off(g, g.n);
-    psec += (int64_t)sig * psec_stride;
+    psi += level_off(g, g.n);
 
-    // Persistent tiles.  Iteration k of workgroup w takes tile k*nwg + xo(w);
-    // xo makes the workgroups of one XCD take consecutive tiles (the section
-    // groups of one box chunk share x lines).  The grid is a multiple of the
-    // number of section groups, so a workgroup keeps the section group
-    // xo % ngroups for all its tiles and loads the section constants once.
+    // Persistent tiles over all nsig signals (tile = signal * ntiles + lid).
+    // Iteration k of workgroup w takes tile k*nwg + xo(w); xo makes the
+    // workgroups of one XCD take consecutive tiles (the section groups of one
+    // box chunk share x lines).  The grid is a multiple of the number of
+    // section groups, so a workgroup keeps the section group xo % ngroups for
+    // all its tiles (of every signal) and loads the section constants once.
     const int ngroups = P / G;
     const int ntiles = ngroups * (g.s / BB);
+    const int ntot = nsig * ntiles;
     const int nwg = gridDim.x, w = blockIdx.x;
     const int xo = (nwg % 8 == 0) ? (w % 8) * (nwg / 8) + w / 8 : w;
     const int sc0 = (xo % ngroups) * G;
@@ -676,7 +675,10 @@ void k_leaf_mma(Geom g, const cx<T> *__restrict__ x, int64_t x_stride,
     // while the previous tile computes).  Every load is unconditional
     // (indices clamped): a lane-conditional load becomes a branch, and the
     // counter wait after a branch covers every load in flight.
-    auto prefetch = [&](int lid) __attribute__((always_inline)) {
+    auto prefetch = [&](int tile) __attribute__((always_inline)) {
+        const int sig = tile / ntiles, lid = tile - sig * ntiles;
+        const cx<T> *xs = x + (int64_t)sig * x_stride;
+        const T *ps = psi + (int64_t)sig * tree_stride;
         const int i0 = (lid / ngroups) * BB;
         const int mu_w0 = (i0 - 2) * b;
 #pragma unroll
@@ -684,7 +686,7 @@ void k_leaf_mma(Geom g, const cx<T> *__restrict__ x, int64_t x_stride,
             const int e = (XN % 256 == 0) ? threadIdx.x + 256 * u : min((int)threadIdx.x + 256 * u, XN - 1);
             const int r = e / G, ge = e % G;
             const int mu = (mu_w0 + r) & mmask;  // periodic window
-            xr[u] = x[(mu << lp) + sc0 + ge];
+            xr[u] = xs[(mu << lp) + sc0 + ge];
         }
 #pragma unroll
         for (int u = 0; u < LPER; ++u) {
@@ -694,7 +696,7 @@ void k_leaf_mma(Geom g, const cx<T> *__restrict__ x, int64_t x_stride,
             const int sce = sc0 + (rr >> 1);
             const int box = (i0 - 1 + bx9) & smask;
             // padding terms meet zero L2T rows, section 0 never reads its slot
-            lr[u] = psi[(box * terms + min(kk, terms - 1)) * rowsS + max(2 * (sce - 1) + (rr & 1), 0)];
+            lr[u] = ps[(box * terms + min(kk, terms - 1)) * rowsS + max(2 * (sce - 1) + (rr & 1), 0)];
         }
     };
     auto commit = [&]() __attribute__((always_inline)) {
@@ -715,22 +717,24 @@ void k_leaf_mma(Geom g, const cx<T> *__restrict__ x, int64_t x_stride,
     };
 
     int k = 0;
-    int lid = xo;
-    if (lid < ntiles) prefetch(lid);
-    while (lid < ntiles) {
+    int tile = xo;
+    if (tile < ntot) prefetch(tile);
+    while (tile < ntot) {
         __syncthreads();  // the previous tile is no longer read
         commit();
         __syncthreads();
+        const int sig = tile / ntiles, lid = tile - sig * ntiles;
+        cx<T> *psec_s = psec + (int64_t)sig * psec_stride;
         const int i0 = (lid / ngroups) * BB;
         // past the last tile the prefetch re-reads the current one (still in
         // L2; an unconditional prefetch keeps the counter waits exact)
-        const int nlid = (++k) * nwg + xo;
-        prefetch(nlid < ntiles ? nlid : lid);
+        const int ntile = (++k) * nwg + xo;
+        prefetch(ntile < ntot ? ntile : tile);
 
         if (gg < G) {
             if (sc == 0) {
                 for (int t = lane; t < BB * b; t += 64)
-                    psec[rm(i0 * b + t)] = xw[(2 * b + t) * G + boxpad((2 * b + t) / b)];
+                    psec_s[rm(i0 * b + t)] = xw[(2 * b + t) * G + boxpad((2 * b + t) / b)];
             } else {
                 const T *kr_ = kvs + gg * 4 * b;
                 V4 acc[NRT][CT];
@@ -838,7 +842,7 @@ void k_leaf_mma(Geom g, const cx<T> *__restrict__ x, int64_t x_stride,
                 for (int ct = 0; ct < CT; ++ct)
 #pragma unroll
                 for (int rt = 0; rt < NRT; ++rt) {
-                    cx<T> *dst = psec + (int64_t)sc * M;
+                    cx<T> *dst = psec_s + (int64_t)sc * M;
                     const int mu = (i0 + bx + 8 * ct) * b + 16 * rt;
                     const T a0 = acc[rt][ct][0], a1 = acc[rt][ct][1], a2 = acc[rt][ct][2], a3 = acc[rt][ct][3];
                     const T keep0 = bsel(c, a2, a0), keep1 = bsel(c, a3, a1);
@@ -858,7 +862,7 @@ void k_leaf_mma(Geom g, const cx<T> *__restrict__ x, int64_t x_stride,
                 }
             }
         }
-        lid = nlid;
+        tile = ntile;
     }
 }
 
