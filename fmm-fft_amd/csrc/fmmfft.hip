@@ -313,16 +313,17 @@ void fft_form(const Geom &g, const FftPass &fp, unsigned nz, hipStream_t st, con
               int64_t in_stride, int64_t out_stride, const cx<T> *tw, const c64 *tlo, const c64 *thi,
               bool force_single) {
     constexpr int C = fft_cols<T, L>();
-    const int64_t ntiles = (fp.F * g.P + C - 1) / C;
+    // tiles of all nz signals (a batch is one persistent grid)
+    const int64_t ntot = (fp.F * g.P + C - 1) / C * (int64_t)nz;
     const dim3 block(kFftThreads);
     const unsigned res =
-        force_single ? (unsigned)ntiles : resident_grid((const void *)k_fft<T, L, FORM, C, true>, kFftThreads, 0, ntiles);
-    if ((int64_t)res < ntiles)
-        hipLaunchKernelGGL((k_fft<T, L, FORM, C, true>), dim3(res, 1, nz), block, 0, st, g, fp, in, out, in_stride,
-                           out_stride, tw, tlo, thi);
+        force_single ? (unsigned)ntot : resident_grid((const void *)k_fft<T, L, FORM, C, true>, kFftThreads, 0, ntot);
+    if ((int64_t)res < ntot)
+        hipLaunchKernelGGL((k_fft<T, L, FORM, C, true>), dim3(res, 1, 1), block, 0, st, g, fp, in, out, in_stride,
+                           out_stride, tw, tlo, thi, (int)nz);
     else
-        hipLaunchKernelGGL((k_fft<T, L, FORM, C, false>), dim3((unsigned)ntiles, 1, nz), block, 0, st, g, fp, in,
-                           out, in_stride, out_stride, tw, tlo, thi);
+        hipLaunchKernelGGL((k_fft<T, L, FORM, C, false>), dim3((unsigned)ntot, 1, 1), block, 0, st, g, fp, in, out,
+                           in_stride, out_stride, tw, tlo, thi, (int)nz);
 }
 
 template <typename T, int L>

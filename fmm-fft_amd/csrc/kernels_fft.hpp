@@ -218,19 +218,17 @@ template <typename T, int L, int FORM, int C = fft_cols<T, L>(), bool PERSIST = 
 __global__ __launch_bounds__(kFftThreads) void k_fft(Geom g, FftPass fp, const cx<T> *__restrict__ in,
                                              cx<T> *__restrict__ out, int64_t in_stride, int64_t out_stride,
                                              const cx<T> *__restrict__ twL, const c64 *__restrict__ tlo,
-                                             const c64 *__restrict__ thi) {
+                                             const c64 *__restrict__ thi, int nsig) {
     constexpr int ls = tile_stride<L, C, (int)sizeof(cx<T>)>();
     constexpr int n = C * L;
     constexpr int NT = kFftThreads;
     constexpr int PER = n / NT;
     static_assert(n % NT == 0 && NT % C == 0, "tile shape");
     __shared__ cx<T> a[C * ls];
-    const int sig = blockIdx.z;
-    in += (int64_t)sig * in_stride;
-    out += (int64_t)sig * out_stride;
     const int M = (int)g.M, F = (int)fp.F;
     const int nf = F * (int)g.P;  // sub-FFTs per signal
     const int ntiles = (nf + C - 1) / C;
+    const int ntot = nsig * ntiles;  // tiles of all signals: tile = signal * ntiles + t
     const int nwg = gridDim.x;
     const int xo = (int)xcd_remap(blockIdx.x, nwg);
     // when F == 1 the C sub-FFTs of a tile are C consecutive blocks
@@ -246,7 +244,9 @@ __global__ __launch_bounds__(kFftThreads) void k_fft(Geom g, FftPass fp, const c
     // re/im in separate scalar arrays and unconditional loads (a ragged last
     // tile re-reads its last valid sub-FFT): the prefetch stays in registers.
     T vr[PER], vi[PER];
-    auto prefetch = [&](int t) __attribute__((always_inline)) {
+    auto prefetch = [&](int tt) __attribute__((always_inline)) {
+        const int sig = tt / ntiles, t = tt - sig * ntiles;
+        const cx<T> *in_s = in + (int64_t)sig * in_stride;
         const int fg0 = t * C;
         const int cl = min(C, nf - fg0) - 1;
         const int k2 = fg0 / F, f0 = fg0 - k2 * F, base = k2 * M;
@@ -255,15 +255,15 @@ __global__ __launch_bounds__(kFftThreads) void k_fft(Geom g, FftPass fp, const c
             const int i = threadIdx.x + u * NT;
             const int f = erow ? i / L : i % C, e = erow ? i % L : i / C;
             const int fo = (f0 >> in_ltw) * fblk_in + ((f0 & ((1 << in_ltw) - 1)) + min(f, cl)) * fstep_in;
-            const cx<T> x = in[base + fo + e * in_es];
+            const cx<T> x = in_s[base + fo + e * in_es];
             vr[u] = x.re;
             vi[u] = x.im;
         }
     };
 
-    int t = xo;
-    if (t < ntiles) prefetch(t);
-    while (t < ntiles) {
+    int tt = xo;
+    if (tt < ntot) prefetch(tt);
+    while (tt < ntot) {
         __syncthreads();  // the previous tile's outputs have left LDS
 #pragma unroll
         for (int u = 0; u < PER; ++u) {
@@ -272,6 +272,8 @@ __global__ __launch_bounds__(kFftThreads) void k_fft(Geom g, FftPass fp, const c
             a[f * ls + lpad(e)] = cx<T>{vr[u], vi[u]};
         }
         __syncthreads();
+        const int sig = tt / ntiles, t = tt - sig * ntiles;
+        cx<T> *out_s = out + (int64_t)sig * out_stride;
         const int fg0 = t * C;
         const int cv = min(C, nf - fg0);
         const int k2 = fg0 / F, f0 = fg0 - k2 * F, base = k2 * M;
@@ -295,8 +297,8 @@ __global__ __launch_bounds__(kFftThreads) void k_fft(Geom g, FftPass fp, const c
         // past the last tile the prefetch re-reads the current one (still in
         // L2, and the prefetch stays unconditional, which keeps the
         // counter waits exact)
-        const int tn = t + nwg;
-        if constexpr (PERSIST) prefetch(tn < ntiles ? tn : t);
+        const int tn = tt + nwg;
+        if constexpr (PERSIST) prefetch(tn < ntot ? tn : tt);
         stockham<T, L, C, 1, NT>(a, ls, twL);
 
         c64 w{1.0, 0.0}, ws{1.0, 0.0};
@@ -321,9 +323,9 @@ __global__ __launch_bounds__(kFftThreads) void k_fft(Geom g, FftPass fp, const c
                 w = cmul(w, ws);
             }
             if (fp.conj_out) val.im = -val.im;
-            if (f < cv) out[base + (f0 + f) * fstep_out + k * out_ks] = val;
+            if (f < cv) out_s[base + (f0 + f) * fstep_out + k * out_ks] = val;
         }
-        t = tn;
+        tt = tn;
     }
 }
 
