@@ -73,12 +73,18 @@ struct RowMap {
 // padded to 4 * leaf_kt(terms) terms.
 __host__ __device__ inline int leaf_kt(int terms) { return terms <= 16 ? 4 : 8; }
 
-// Real offset of level l (levels 2..n, 2^l nodes of [terms][S][2] reals).
+// Tree nodes are [terms][2P] reals: row 2 sc + re/im of section sc.
+// Section 0 has no cot matvec, but its rows are kept (they carry its
+// expansion, which nothing reads): a term row is then 2P reals, a multiple
+// of 64 bytes for P >= 8, and a block of 16 rows is 8 whole sections whose
+// reals sit at the same offsets as in a row of x.
+__host__ __device__ inline int tree_rows(const Geom &g) { return 2 * g.P; }
+// Real offset of level l (levels 2..n, 2^l nodes).
 __host__ __device__ inline int64_t level_off(const Geom &g, int l) {
-    return ((int64_t(1) << l) - 4) * int64_t(g.S) * 2 * g.terms;
+    return ((int64_t(1) << l) - 4) * int64_t(tree_rows(g)) * g.terms;
 }
 __host__ __device__ inline int64_t tree_len(const Geom &g) {
-    return ((int64_t(1) << (g.n + 1)) - 4) * int64_t(g.S) * 2 * g.terms;
+    return ((int64_t(1) << (g.n + 1)) - 4) * int64_t(tree_rows(g)) * g.terms;
 }
 
 __device__ inline int pmod(int a, int m) {

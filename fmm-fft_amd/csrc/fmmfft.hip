@@ -497,7 +497,7 @@ void s2m_tree_pt(const Geom &g, const Shapes &sh, unsigned nz, hipStream_t st, c
     // tree passes on the matrix cores: terms padded to 16 or 32, one wave per
     // (node, block of 16 rows), four waves per workgroup
     constexpr int TP = PT <= 16 ? 16 : 32;
-    const int64_t nrb = (2 * (int64_t)g.S + 15) / 16;
+    const int64_t nrb = (tree_rows(g) + 15) / 16;
     // nodes of at most two row blocks (P <= 16): packed form, several
     // nodes per wave
     const bool pack = nrb <= 2;
@@ -889,6 +889,15 @@ int fmmfft_read_stage(fmmfft_plan plan, int32_t which, void *host_dst, size_t by
         case 2: off = wl.psi; cnt = (size_t)tree_len(g) * plan->prm.esize; break;
         case 3: off = wl.psec; cnt = (size_t)g.N * 2 * plan->prm.esize; break;
         default: return FMMFFT_EINVAL;
+    }
+    if (which == 1 || which == 2) {
+        // the tree as [node][terms][S][2]: the rows of section 0 are dropped
+        const size_t es = plan->prm.esize, nrows = (size_t)tree_len(g) / tree_rows(g);
+        if (bytes != nrows * 2 * (size_t)g.S * es) return FMMFFT_EINVAL;
+        HIP_TRY(hipDeviceSynchronize());
+        HIP_TRY(hipMemcpy2D(host_dst, 2 * (size_t)g.S * es, (char *)plan->work + off + 2 * es,
+                            (size_t)tree_rows(g) * es, 2 * (size_t)g.S * es, nrows, hipMemcpyDeviceToHost));
+        return FMMFFT_OK;
     }
     if (bytes != cnt) return FMMFFT_EINVAL;
     HIP_TRY(hipDeviceSynchronize());

@@ -6,9 +6,11 @@
 //   x     [M][P]  complex, natural order: section sc is column sc
 //                 (the reference's polyphase permute, src/fmmfft.c:419-439,
 //                 is only an index map here)
-//   tree  levels 2..n, level l has 2^l nodes of [terms][P-1][2] reals:
+//   tree  levels 2..n, level l has 2^l nodes of [terms][P][2] reals:
 //                 term-major, rows (section, re/im) innermost, so lanes that
-//                 own consecutive rows access consecutive addresses
+//                 own consecutive rows access consecutive addresses; the
+//                 rows of section 0 are carried (never read) so that a
+//                 term row is 2P reals and row blocks stay aligned
 //   psec  [P][M]  complex: row sc >= 1 is the cot sum of section sc,
 //                 row 0 is a copy of x_0
 //   y     [P][M]  complex: block k2 feeds the k2-th length-M FFT
@@ -51,7 +53,7 @@ __global__ __launch_bounds__(kS2mThreads) void k_s2m(Geom g, const cx<T> *__rest
     extern __shared__ __align__(16) unsigned char smem[];
     T *ich = reinterpret_cast<T *>(smem);  // [b][PT], zero padded
     const int P = g.P, b = g.b, terms = g.terms;
-    const int rows = 2 * g.S;
+    const int rows = tree_rows(g);
     const int bx_n = min(TILE / (b * P), g.s);          // boxes per tile
     const int tile_elems = bx_n * b * P, tile_vecs = tile_elems * (int)sizeof(cx<T>) / 16;
     const int ntiles = g.s / bx_n;
@@ -119,8 +121,8 @@ __global__ __launch_bounds__(kS2mThreads) void k_s2m(Geom g, const cx<T> *__rest
                 sre += pre;
                 sim += pim;
             }
-            if (sc > 0) {
-                T *dst = phi + (int64_t)(t * bx_n + bx) * terms * rows + 2 * (sc - 1);
+            {
+                T *dst = phi + (int64_t)(t * bx_n + bx) * terms * rows + 2 * sc;
 #pragma unroll
                 for (int k = 0; k < 4; ++k)
                     if (4 * kg + k < terms) {
@@ -201,7 +203,7 @@ __global__ __launch_bounds__(256) void k_leaf(Geom g, const cx<T> *__restrict__ 
                                               const T *__restrict__ evalm, const T *__restrict__ evalmh,
                                               const T *__restrict__ kv, int G, int bb, int conj, RowMap rm) {
     extern __shared__ __align__(16) unsigned char smem[];
-    const int P = g.P, b = g.b, terms = g.terms, S = g.S;
+    const int P = g.P, b = g.b, terms = g.terms;
     const int64_t M = g.M;
     const int rows = (bb + 3) * b;
     const int kvl = 4 * b + 1;  // padded row of the kernel generator
@@ -237,7 +239,7 @@ __global__ __launch_bounds__(256) void k_leaf(Geom g, const cx<T> *__restrict__ 
         const int sc = sc0 + gg;
         const int box = pmod(i0 - 1 + bx, g.s);
         const int cc = t / terms, k = t - cc * terms;
-        loc[e] = sc == 0 ? T(0) : psi[((int64_t)box * terms + k) * (2 * S) + 2 * (sc - 1) + cc];
+        loc[e] = sc == 0 ? T(0) : psi[((int64_t)box * terms + k) * tree_rows(g) + 2 * sc + cc];
     }
     __syncthreads();
 
@@ -415,7 +417,7 @@ __global__ __launch_bounds__(256) void k_s2m_mma(Geom g, const cx<T> *__restrict
     using VT = typename VecOf<T, VEC>::type;
     constexpr int TP = 16 * MT, KT = 4 * MT, FR = TP * TP;  // the tree's padded terms and K steps
     __shared__ double red[4][16 * VEC];
-    const int P = g.P, terms = g.terms, rows = 2 * g.S;
+    const int P = g.P, terms = g.terms, rows = tree_rows(g);
     const int nchunk = 2 * P / (16 * VEC), pper = 4 / nchunk;  // chunks of a row, box pairs per workgroup step
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, col = lane & 15, kr = lane >> 4;
     const int chunk = wave % nchunk, pslot = wave / nchunk;
@@ -485,12 +487,12 @@ __global__ __launch_bounds__(256) void k_s2m_mma(Geom g, const cx<T> *__restrict
                 for (int m = 0; m < MT; ++m) acc[m][e] = Mfma<T>::mma(a[m][H * half + s], bv, acc[m][e]);
             }
     };
-    // node [k][n - 2] of a level; section 0 (no tree rows) and the padding
-    // terms k >= terms go to the sink, so every store is unconditional (a
-    // store under a lane condition becomes a branch, and the counter wait
-    // after a branch covers the loads in flight)
+    // node [k][n] of a level (tree rows are the reals of an x row); the
+    // padding terms k >= terms go to the sink, so every store is
+    // unconditional (a store under a lane condition becomes a branch, and
+    // the counter wait after a branch covers the loads in flight)
     auto store = [&](T *node, const V4 (&acc)[MT][VEC]) __attribute__((always_inline)) {
-        T *dst = node + n0 - 2;
+        T *dst = node + n0;
 #pragma unroll
         for (int m = 0; m < MT; ++m)
 #pragma unroll
@@ -499,7 +501,7 @@ __global__ __launch_bounds__(256) void k_s2m_mma(Geom g, const cx<T> *__restrict
                 T *d = dst + (int64_t)k * rows;
 #pragma unroll
                 for (int e = 0; e < VEC; e += 2) {
-                    T *de = (k < terms && n0 + e >= 2) ? d + e : sink;
+                    T *de = k < terms ? d + e : sink;
                     *reinterpret_cast<float2 *>(de) =
                         make_float2(sgn[e] * acc[m][e][r], sgn[e + 1] * acc[m][e + 1][r]);
                 }
@@ -606,9 +608,9 @@ void k_leaf_mma(Geom g, const cx<T> *__restrict__ x, int64_t x_stride,
     // re-homed target, row of target 0; zero past terms)
     constexpr int EN = G * 2 * TP;
     extern __shared__ __align__(16) unsigned char smem[];
-    const int P = g.P, terms = g.terms, S = g.S;
+    const int P = g.P, terms = g.terms;
     const int64_t M = g.M;
-    const int rowsS = 2 * S;             // tree row stride
+    const int rowsS = tree_rows(g);      // tree row stride
     // offsets inside one signal fit 32 bits (N <= 2^26); P, M and s are
     // powers of two
     const int lp = __builtin_ctz(P), mmask = (int)M - 1, smask = g.s - 1;
@@ -695,8 +697,9 @@ void k_leaf_mma(Geom g, const cx<T> *__restrict__ x, int64_t x_stride,
             const int kk = rem / (2 * G), rr = rem % (2 * G);
             const int sce = sc0 + (rr >> 1);
             const int box = (i0 - 1 + bx9) & smask;
-            // padding terms meet zero L2T rows, section 0 never reads its slot
-            lr[u] = ps[(box * terms + min(kk, terms - 1)) * rowsS + max(2 * (sce - 1) + (rr & 1), 0)];
+            // padding terms meet zero L2T rows; section 0 reads its own rows
+            // and never uses them
+            lr[u] = ps[(box * terms + min(kk, terms - 1)) * rowsS + 2 * sce + (rr & 1)];
         }
     };
     auto commit = [&]() __attribute__((always_inline)) {
@@ -917,7 +920,7 @@ __global__ __launch_bounds__(256) void k_tree_mma(Geom g, int l, const T *__rest
     constexpr int NT = MODE == 0 ? 2 : MODE == 1 ? 4 : 1;  // sources per node
     constexpr int RB = PACK ? tree_mma_rb_packed<T, TP>(MODE) : tree_mma_rb<T, TP>(MODE);
     constexpr int FR = TP * TP;                             // one operator in fragment order
-    const int terms = g.terms, rows = 2 * g.S;
+    const int terms = g.terms, rows = tree_rows(g);
     const int node_len = rows * terms;
     const int nn = 1 << l, nrb = (rows + 15) >> 4, nw = (nrb + RB - 1) / RB;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
