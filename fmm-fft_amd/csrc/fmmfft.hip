@@ -131,6 +131,7 @@ TableLayout table_layout(const Geom &g, const FftLayout &f) {
 // Launch shapes of the leaf and S2M kernels.
 struct Shapes {
     int s2m_bpb, s2m_nblk;
+    int s2m_levels = 0;  // tree levels above the leaves the S2M kernel forms
     size_t s2m_lds;
     int s2m_vec;  // matrix-core S2M: reals per lane load (0: vector-FMA kernel)
     RowMap rm;    // row order of the leaf output (column-tiled for two FFT passes)
@@ -189,8 +190,13 @@ Shapes shapes(const Geom &g, size_t esize) {
         }
         sh.s2m_vec = vec;
         if (vec) {
+            // sibling quads (levels n-1 and n-2 formed too) when the tree
+            // has them and the terms fit one 16-row block (with 32 the quad
+            // form holds one wave per SIMD), else sibling pairs
             const int nchunk = 2 * g.P / (16 * vec), pper = 4 / nchunk;
-            sh.s2m_nblk = std::min((g.s / 2 + pper - 1) / pper, 768);
+            sh.s2m_levels = (g.n >= 4 && g.terms <= 16) ? 2 : 1;
+            const int units = g.s >> sh.s2m_levels;
+            sh.s2m_nblk = std::min((units + pper - 1) / pper, 768);
         }
     }
 
@@ -447,20 +453,33 @@ void s2m_mma_ks(const Geom &g, const Shapes &sh, unsigned nz, hipStream_t st, co
                 int64_t wstride_T, c64 *sums_part, int64_t wstride_c64, const T *initch, const T *opfrag,
                 c64 *sumsec, int conj) {
     const dim3 grid(sh.s2m_nblk, 1, nz);
+    const bool quad = sh.s2m_levels == 2;
     T *sink = reinterpret_cast<T *>(sumsec);
     const int64_t sink_stride = wstride_c64 * 16 / (int64_t)sizeof(T);
     switch (g.b) {
         case 16:
-            hipLaunchKernelGGL((k_s2m_mma<T, VEC, MT, 4>), grid, dim3(256), 0, st, g, x, g.N, phi, wstride_T,
-                               sums_part, wstride_c64, initch, opfrag, sink, sink_stride, conj);
+            if (quad)
+                hipLaunchKernelGGL((k_s2m_mma<T, VEC, MT, 4, true>), grid, dim3(256), 0, st, g, x, g.N, phi,
+                                   wstride_T, sums_part, wstride_c64, initch, opfrag, sink, sink_stride, conj);
+            else
+                hipLaunchKernelGGL((k_s2m_mma<T, VEC, MT, 4, false>), grid, dim3(256), 0, st, g, x, g.N, phi,
+                                   wstride_T, sums_part, wstride_c64, initch, opfrag, sink, sink_stride, conj);
             break;
         case 32:
-            hipLaunchKernelGGL((k_s2m_mma<T, VEC, MT, 8>), grid, dim3(256), 0, st, g, x, g.N, phi, wstride_T,
-                               sums_part, wstride_c64, initch, opfrag, sink, sink_stride, conj);
+            if (quad)
+                hipLaunchKernelGGL((k_s2m_mma<T, VEC, MT, 8, true>), grid, dim3(256), 0, st, g, x, g.N, phi,
+                                   wstride_T, sums_part, wstride_c64, initch, opfrag, sink, sink_stride, conj);
+            else
+                hipLaunchKernelGGL((k_s2m_mma<T, VEC, MT, 8, false>), grid, dim3(256), 0, st, g, x, g.N, phi,
+                                   wstride_T, sums_part, wstride_c64, initch, opfrag, sink, sink_stride, conj);
             break;
         default:
-            hipLaunchKernelGGL((k_s2m_mma<T, VEC, MT, 16>), grid, dim3(256), 0, st, g, x, g.N, phi, wstride_T,
-                               sums_part, wstride_c64, initch, opfrag, sink, sink_stride, conj);
+            if (quad)
+                hipLaunchKernelGGL((k_s2m_mma<T, VEC, MT, 16, true>), grid, dim3(256), 0, st, g, x, g.N, phi,
+                                   wstride_T, sums_part, wstride_c64, initch, opfrag, sink, sink_stride, conj);
+            else
+                hipLaunchKernelGGL((k_s2m_mma<T, VEC, MT, 16, false>), grid, dim3(256), 0, st, g, x, g.N, phi,
+                                   wstride_T, sums_part, wstride_c64, initch, opfrag, sink, sink_stride, conj);
             break;
     }
 }
@@ -517,8 +536,8 @@ void s2m_tree_pt(const Geom &g, const Shapes &sh, unsigned nz, hipStream_t st, c
         hipLaunchKernelGGL(pack ? pkern : kern, grid(l, mode), dim3(256), 0, st, g, l, (const T *)phi, phi,
                            (const T *)psi, psi, wstride_T, tb.opfrag);
     };
-    // the matrix-core S2M has already formed level n-1
-    for (int l = g.n - 1 - (sh.s2m_vec ? 1 : 0); l >= 2; --l)
+    // the matrix-core S2M has already formed levels n-1 (and n-2)
+    for (int l = g.n - 1 - (sh.s2m_vec ? sh.s2m_levels : 0); l >= 2; --l)
         level(k_tree_mma<T, TP, 0, false>, k_tree_mma<T, TP, 0, true>, l, 0);
     mark(2);
     level(k_tree_mma<T, TP, 2, false>, k_tree_mma<T, TP, 2, true>, 2, 2);

@@ -398,7 +398,9 @@ struct Mfma<double> {
 // whole kernel) and adds its reals into FP64 section sums; no LDS until the
 // final per-workgroup reduction of the sums.  Boxes come in sibling pairs
 // and the first upward level (M2M to level n-1, src/fmmfft.c:476-515) runs
-// on the pair's accumulators before they leave registers.
+// on the pair's accumulators before they leave registers; with QUAD (tree
+// depth >= 4) a wave takes sibling quads and also forms level n-2 from the
+// two parents.
 // ---------------------------------------------------------------------------
 template <typename T, int VEC>
 struct VecOf;
@@ -407,7 +409,7 @@ template <> struct VecOf<float, 2> { typedef float2 type; };
 __device__ inline float velem(const float4 &v, int e) { return e == 0 ? v.x : e == 1 ? v.y : e == 2 ? v.z : v.w; }
 __device__ inline float velem(const float2 &v, int e) { return e == 0 ? v.x : v.y; }
 
-template <typename T, int VEC, int MT, int KS>
+template <typename T, int VEC, int MT, int KS, bool QUAD>
 __global__ __launch_bounds__(256) void k_s2m_mma(Geom g, const cx<T> *__restrict__ x, int64_t x_stride,
                                                  T *__restrict__ phi, int64_t tree_stride,
                                                  c64 *__restrict__ sums_part, int64_t sums_stride,
@@ -443,9 +445,11 @@ __global__ __launch_bounds__(256) void k_s2m_mma(Geom g, const cx<T> *__restrict
     // operator fragments are read with that K order: am[c][ko][s] =
     // A_c[16 ko + col][16 (s/4) + 4 kr + s%4] from the fragment-ordered table
     // (element A[i][j] at ((j/4) KO + ko) 64 + i%16 + 16 (j%4))
-    T am[2][MT][KT];
-    {
-        const T *o = opfrag + (size_t)(g.n - 3) * NOPS * FR;
+    // (QUAD: the same for level n-2 from the two parents, am2)
+    T am[2][MT][KT], am2[QUAD ? 2 : 1][MT][KT];
+#pragma unroll
+    for (int lv = 0; lv < (QUAD ? 2 : 1); ++lv) {
+        const T *o = opfrag + (size_t)(g.n - 3 - lv) * NOPS * FR;
         const T *op[2] = {o + OP_SHFTFN * FR, o + OP_SHFTFP * FR};
 #pragma unroll
         for (int c2 = 0; c2 < 2; ++c2)
@@ -454,7 +458,9 @@ __global__ __launch_bounds__(256) void k_s2m_mma(Geom g, const cx<T> *__restrict
 #pragma unroll
                 for (int s = 0; s < KT; ++s) {
                     const int j = 16 * (s / 4) + 4 * kr + (s % 4);
-                    am[c2][ko][s] = op[c2][((j / 4) * MT + ko) * 64 + col + 16 * (j % 4)];
+                    const T v = op[c2][((j / 4) * MT + ko) * 64 + col + 16 * (j % 4)];
+                    if (lv == 0) am[c2][ko][s] = v;
+                    else if constexpr (QUAD) am2[c2][ko][s] = v;
                 }
     }
     // the imaginary parts change sign for the conjugated (inverse) input
@@ -513,12 +519,24 @@ __global__ __launch_bounds__(256) void k_s2m_mma(Geom g, const cx<T> *__restrict
 #pragma unroll
             for (int e = 0; e < VEC; ++e) acc[m][e] = V4{0, 0, 0, 0};
     };
-    const int64_t npairs = g.s / 2, step = (int64_t)gridDim.x * pper;
+    // parent = shftfn . child 0 + shftfp . child 1, on accumulators in the
+    // D layout (the operators were read in that K order)
+    auto m2m = [&](V4 (&dst)[MT][VEC], const T (&op)[MT][KT], const V4 (&ch)[MT][VEC])
+                   __attribute__((always_inline)) {
+#pragma unroll
+        for (int s = 0; s < KT; ++s)
+#pragma unroll
+            for (int e = 0; e < VEC; ++e) {
+                const T bv = ch[s / 4][e][s % 4];
+#pragma unroll
+                for (int ko = 0; ko < MT; ++ko) dst[ko][e] = Mfma<T>::mma(op[ko][s], bv, dst[ko][e]);
+            }
+    };
     const int64_t node_len = (int64_t)terms * rows;
-    int64_t q = (int64_t)blockIdx.x * pper + pslot;
-    VT va[H], vb[H];
-    if (q < npairs) load(va, 2 * q, 0);
-    while (q < npairs) {
+    // one sibling pair (2q, 2q+1): leaves, then parent q into up; va holds
+    // the first half of box 2q on entry and the first half of box nb on exit
+    auto pair = [&](int64_t q, int64_t nb, VT (&va)[H], VT (&vb)[H], V4 (&up)[MT][VEC])
+                    __attribute__((always_inline)) {
         V4 acc[2][MT][VEC];
         load(vb, 2 * q, 1);
         zero(acc[0]);
@@ -529,25 +547,42 @@ __global__ __launch_bounds__(256) void k_s2m_mma(Geom g, const cx<T> *__restrict
         load(vb, 2 * q + 1, 1);
         zero(acc[1]);
         mma_half(acc[1], va, 0);
-        const int64_t nq = q + step;
-        load(va, 2 * (nq < npairs ? nq : q), 0);  // past the end: this pair again (in L2)
+        load(va, nb, 0);
         mma_half(acc[1], vb, 1);
         store(phi_n + (2 * q + 1) * node_len, acc[1]);
-        // parent q = shftfn . child 2q + shftfp . child 2q+1
-        V4 up[MT][VEC];
         zero(up);
-#pragma unroll
-        for (int c2 = 0; c2 < 2; ++c2)
-#pragma unroll
-            for (int s = 0; s < KT; ++s)
-#pragma unroll
-                for (int e = 0; e < VEC; ++e) {
-                    const T bv = acc[c2][s / 4][e][s % 4];
-#pragma unroll
-                    for (int ko = 0; ko < MT; ++ko) up[ko][e] = Mfma<T>::mma(am[c2][ko][s], bv, up[ko][e]);
-                }
+        m2m(up, am[0], acc[0]);
+        m2m(up, am[1], acc[1]);
         store(phi_p + q * node_len, up);
-        q = nq;
+    };
+    VT va[H], vb[H];
+    if constexpr (!QUAD) {
+        const int64_t npairs = g.s / 2, step = (int64_t)gridDim.x * pper;
+        int64_t q = (int64_t)blockIdx.x * pper + pslot;
+        if (q < npairs) load(va, 2 * q, 0);
+        while (q < npairs) {
+            const int64_t nq = q + step;
+            V4 up[MT][VEC];
+            pair(q, 2 * (nq < npairs ? nq : q), va, vb, up);  // past the end: this pair again (in L2)
+            q = nq;
+        }
+    } else {
+        // sibling quads: two pairs, then their parents' parent (level n-2)
+        T *phi_g = phi_p + level_off(g, g.n - 2) - level_off(g, g.n - 1);
+        const int64_t nquads = g.s / 4, step = (int64_t)gridDim.x * pper;
+        int64_t Q = (int64_t)blockIdx.x * pper + pslot;
+        if (Q < nquads) load(va, 4 * Q, 0);
+        while (Q < nquads) {
+            const int64_t nQ = Q + step;
+            V4 up[MT][VEC], gp[MT][VEC];
+            pair(2 * Q, 4 * Q + 2, va, vb, up);
+            zero(gp);
+            m2m(gp, am2[0], up);
+            pair(2 * Q + 1, 4 * (nQ < nquads ? nQ : Q), va, vb, up);  // past the end: this quad again
+            m2m(gp, am2[1], up);
+            store(phi_g + Q * node_len, gp);
+            Q = nQ;
+        }
     }
     // per-section partial sums of this workgroup, fixed order: lane groups
     // (kr), then the waves of each chunk
@@ -900,8 +935,10 @@ __host__ __device__ constexpr int tree_mma_rb_packed(int mode) {
 template <typename T, int TP>
 __host__ __device__ constexpr int tree_mma_rb(int mode) {
     // row blocks per wave: the B fragments of one wave stay within ~64 VGPRs;
-    // the FP32 downward pass measures best at 2 (more waves in flight)
+    // fewer where more waves in flight measured faster (FP32 downward, and
+    // the 16-term upward pass in both precisions)
     if (mode == 1 && sizeof(T) == 4) return 2;
+    if (mode == 0 && TP == 16) return sizeof(T) == 4 ? 1 : 2;
     return mode == 2 ? 4 : (64 * 4 / ((mode == 0 ? 2 : 4) * (TP / 4) * (int)sizeof(T))) >= 4 ? 4
                        : (64 * 4 / ((mode == 0 ? 2 : 4) * (TP / 4) * (int)sizeof(T))) >= 2 ? 2 : 1;
 }
