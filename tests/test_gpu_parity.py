@@ -94,6 +94,39 @@ def test_stages_match_oracle_fp64(fmm, orc, torch, n, p, terms, leaf):
     assert np.array_equal(g_ps[:, 0], x.reshape(M, p)[:, 0])
 
 
+# complex64 plans with 2P >= 32 run the matrix-core S2M, which also forms
+# tree level n-1 (sibling pairs: depth 3, or more than 16 terms) or levels
+# n-1 and n-2 (sibling quads); every level is checked against the oracle's
+# FP64 tree at FP32 accuracy.
+@pytest.mark.parametrize("n,p,terms,leaf", [(1 << 11, 16, 16, 16),   # depth 3: pairs
+                                            (1 << 12, 16, 12, 16),   # depth 4: quads
+                                            (1 << 14, 64, 16, 32),   # depth 3, 16-byte rows
+                                            (1 << 16, 64, 16, 64),   # depth 4, leaf 64
+                                            (1 << 15, 32, 20, 32)])  # 20 terms: pairs
+def test_stages_match_oracle_fp32(fmm, orc, torch, n, p, terms, leaf):
+    say(f"stages fp32 n={n} p={p} terms={terms} leaf={leaf}")
+    x = orc.normal(n, 8).astype(np.complex64).astype(np.complex128)
+    plan, _ = run_plan(fmm, torch, x, n, p, terms, leaf, fmm.C64)
+    ref = orc.OraclePlan(n, p, terms, leaf)
+    S, M = p - 1, n // p
+    g_ss = plan.read_stage(0, p, np.complex128)
+    assert rel(g_ss[1:], ref.section_sums(x)[1:]) < 1e-12 and g_ss[0] == 0
+    phi = ref.upward(x)
+    psi = ref.downward(phi)
+    tl = ref.tree_len()
+    depth = (M // leaf).bit_length() - 1
+    for which, expect in ((1, phi), (2, psi)):
+        raw = plan.read_stage(which, 2 * tl, np.float32).astype(np.float64).reshape(-1, terms, S, 2)
+        got = (raw[..., 0] + 1j * raw[..., 1]).transpose(0, 2, 1).reshape(-1)
+        for lv in range(2, depth + 1):  # level by level: 2^lv nodes
+            lo, cnt = ref.level_off(lv), (1 << lv) * terms * S
+            assert rel(got[lo:lo + cnt], expect[lo:lo + cnt]) < 1e-5, (which, lv)
+        assert ref.level_off(depth) + (1 << depth) * terms * S == tl
+    ps = ref.evaluate(x, psi).reshape(M, p)
+    g_ps = plan.read_stage(3, n, np.complex64).astype(np.complex128).reshape(p, M).T
+    assert rel(g_ps[:, 1:], ps[:, 1:]) < 1e-5
+
+
 # ---------------------------------------------------------------------------
 # whole transform, small sizes
 # ---------------------------------------------------------------------------
