@@ -270,9 +270,10 @@ def path_roofline(stage_ms, esize=8, peak_tflops=PEAK_FP32_TFLOPS):
     of max(algorithmic flops / matrix peak, algorithmic bytes / HBM peak),
     and its ratio to the measured stage times.  Algorithmic counts (a
     complex x real MAC = 4 flops; a tree level l holds 2^l nodes x S
-    sections x p complex terms): S2M reads x, writes tree levels n and n-1
-    (the first M2M runs in the same kernel); the upward pass reads levels
-    n-1..3 and writes n-2..2; the downward pass reads phi_l and psi_(l-1)
+    sections x p complex terms): S2M reads x, writes tree levels n, n-1
+    and, for p <= 16, n-2 (the first M2M levels run in the same kernel);
+    the upward pass reads the rest of the levels down to 3 and writes down
+    to 2; the downward pass reads phi_l and psi_(l-1)
     and writes psi_l for l = 2..n (one parent L2L and three M2L flips per
     node, one flip at the top); the leaf reads x and psi_n and writes the
     section sums; the combine and each FFT pass read and write N points."""
@@ -285,10 +286,12 @@ def path_roofline(stage_ms, esize=8, peak_tflops=PEAK_FP32_TFLOPS):
     m1 = 1 << ((int(math.log2(M)) + 1) // 2)
     m2 = M // m1
     nb = N * esize
+    fused = 2 if (p <= 16 and n >= 4) else 1  # M2M levels formed by the S2M kernel
     count = {
-        "s2m+sums": (4.0 * S * M * p + 8.0 * p * p * S * (s // 2), nb + lvl(n) + lvl(n - 1)),
-        "upward": (8.0 * p * p * S * ((1 << (n - 1)) - 4),
-                   sum(lvl(l + 1) + lvl(l) for l in range(2, n - 1))),
+        "s2m+sums": (4.0 * S * M * p + 8.0 * p * p * S * sum(s >> f for f in range(1, fused + 1)),
+                     nb + sum(lvl(n - f) for f in range(fused + 1))),
+        "upward": (8.0 * p * p * S * ((1 << (n - fused)) - 4),
+                   sum(lvl(l + 1) + lvl(l) for l in range(2, n - fused))),
         "downward": (16.0 * p * p * S * sum(1 << l for l in range(3, n + 1)) + 4.0 * p * p * S * 4,
                      sum(2 * lvl(l) + lvl(l - 1) for l in range(3, n + 1)) + 2 * lvl(2)),
         "leaf": (4.0 * S * M * (3 * LEAF + p), nb + lvl(n) + nb),
