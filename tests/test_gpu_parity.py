@@ -96,12 +96,15 @@ def test_stages_match_oracle_fp64(fmm, orc, torch, n, p, terms, leaf):
 
 # complex64 plans with 2P >= 32 run the matrix-core S2M, which also forms
 # tree level n-1 (sibling pairs: depth 3, or more than 16 terms) or levels
-# n-1 and n-2 (sibling quads); every level is checked against the oracle's
+# n-1 and n-2 (sibling quads), and (P >= 32, 16 terms) the downward levels
+# 3..n-2 run two per launch; every level is checked against the oracle's
 # FP64 tree at FP32 accuracy.
 @pytest.mark.parametrize("n,p,terms,leaf", [(1 << 11, 16, 16, 16),   # depth 3: pairs
                                             (1 << 12, 16, 12, 16),   # depth 4: quads
                                             (1 << 14, 64, 16, 32),   # depth 3, 16-byte rows
                                             (1 << 16, 64, 16, 64),   # depth 4, leaf 64
+                                            (1 << 18, 64, 16, 64),   # depth 6: downward pair (3, 4)
+                                            (1 << 18, 32, 16, 64),   # depth 7: level 3 alone, pair (4, 5)
                                             (1 << 15, 32, 20, 32)])  # 20 terms: pairs
 def test_stages_match_oracle_fp32(fmm, orc, torch, n, p, terms, leaf):
     say(f"stages fp32 n={n} p={p} terms={terms} leaf={leaf}")
@@ -122,9 +125,10 @@ def test_stages_match_oracle_fp32(fmm, orc, torch, n, p, terms, leaf):
             lo, cnt = ref.level_off(lv), (1 << lv) * terms * S
             assert rel(got[lo:lo + cnt], expect[lo:lo + cnt]) < 1e-5, (which, lv)
         assert ref.level_off(depth) + (1 << depth) * terms * S == tl
-    ps = ref.evaluate(x, psi).reshape(M, p)
-    g_ps = plan.read_stage(3, n, np.complex64).astype(np.complex128).reshape(p, M).T
-    assert rel(g_ps[:, 1:], ps[:, 1:]) < 1e-5
+    if M <= 1024:  # larger M reuses the psec buffer in the local FFT
+        ps = ref.evaluate(x, psi).reshape(M, p)
+        g_ps = plan.read_stage(3, n, np.complex64).astype(np.complex128).reshape(p, M).T
+        assert rel(g_ps[:, 1:], ps[:, 1:]) < 1e-5
 
 
 # ---------------------------------------------------------------------------
