@@ -537,8 +537,16 @@ void s2m_tree_pt(const Geom &g, const Shapes &sh, unsigned nz, hipStream_t st, c
                            (const T *)psi, psi, wstride_T, tb.opfrag);
     };
     // the matrix-core S2M has already formed levels n-1 (and n-2)
-    for (int l = g.n - 1 - (sh.s2m_vec ? sh.s2m_levels : 0); l >= 2; --l)
-        level(k_tree_mma<T, TP, 0, false>, k_tree_mma<T, TP, 0, true>, l, 0);
+    int lu = g.n - 1 - (sh.s2m_vec ? sh.s2m_levels : 0);
+    if constexpr (sizeof(T) == 4 && TP == 16) {
+        // complex64, 16 terms, nodes of more than two row blocks: levels
+        // (l, l - 1) two per launch
+        if (!pack)
+            for (; lu - 1 >= 2; lu -= 2)
+                hipLaunchKernelGGL((k_tree_up2<T, TP>), dim3((unsigned)(((int64_t(1) << (lu - 1)) * nrb + 3) / 4), 1, nz),
+                                   dim3(256), 0, st, g, lu, phi, wstride_T, tb.opfrag);
+    }
+    for (; lu >= 2; --lu) level(k_tree_mma<T, TP, 0, false>, k_tree_mma<T, TP, 0, true>, lu, 0);
     mark(2);
     level(k_tree_mma<T, TP, 2, false>, k_tree_mma<T, TP, 2, true>, 2, 2);
     int l = 3;

@@ -1105,6 +1105,76 @@ __global__ __launch_bounds__(256) void k_tree_mma(Geom g, int l, const T *__rest
     }
 }
 
+// Two upward levels in one launch (complex64, 16-row-padded terms): the
+// wave of (node q of level l-1, row block rb) forms its two children
+// phi_l[2q + h] from their children phi_{l+1}[4q + 2h + {0, 1}] and then
+// phi_{l-1}[q] from the two accumulators (as B operands in the D layout's K
+// order), storing all three (src/fmmfft.c:476-515).
+template <typename T, int TP>
+__global__ __launch_bounds__(256) void k_tree_up2(Geom g, int l, T *__restrict__ phi, int64_t tree_stride,
+                                                  const T *__restrict__ opfrag) {
+    using V4 = typename Mfma<T>::v4;
+    static_assert(TP == 16, "one 16-term block");
+    constexpr int KS = TP / 4, FR = TP * TP;
+    const int terms = g.terms, rows = tree_rows(g);
+    const int node_len = rows * terms;
+    const int nq = 1 << (l - 1), nrb = (rows + 15) >> 4;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int col = lane & 15, kr = lane >> 4;
+    const int wid = 4 * (int)xcd_remap(blockIdx.x, gridDim.x) + wave;
+    const int q = wid / nrb, rb = wid - q * nrb;
+    if (q >= nq) return;
+    const int64_t ts = (int64_t)blockIdx.z * tree_stride;
+    const int rc = min(16 * rb + col, rows - 1);
+    const T *ch = phi + ts + level_off(g, l + 1) + (int64_t)(4 * q) * node_len;
+    T *pa = phi + ts + level_off(g, l) + (int64_t)(2 * q) * node_len;
+    T *gp = phi + ts + level_off(g, l - 1) + (int64_t)q * node_len;
+    const T *o1 = opfrag + (size_t)(l - 2) * NOPS * FR, *o2 = opfrag + (size_t)(l - 3) * NOPS * FR;
+    const T *m1[2] = {o1 + OP_SHFTFN * FR, o1 + OP_SHFTFP * FR};
+    const T *m2[2] = {o2 + OP_SHFTFN * FR, o2 + OP_SHFTFP * FR};
+    T a[2][KS], a2[2][KS], bv[4][KS];
+#pragma unroll
+    for (int c2 = 0; c2 < 2; ++c2)
+#pragma unroll
+        for (int s = 0; s < KS; ++s) {
+            a[c2][s] = m1[c2][s * 64 + lane];
+            const int j = 16 * (s / 4) + Mfma<T>::row(kr, s % 4);  // D-as-B K order
+            a2[c2][s] = m2[c2][(j / 4) * 64 + col + 16 * (j % 4)];
+        }
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+#pragma unroll
+        for (int s = 0; s < KS; ++s) bv[k][s] = ch[(int64_t)k * node_len + min(4 * s + kr, terms - 1) * rows + rc];
+    __builtin_amdgcn_sched_barrier(0);
+    V4 up[2] = {V4{0, 0, 0, 0}, V4{0, 0, 0, 0}};
+#pragma unroll
+    for (int c2 = 0; c2 < 2; ++c2)
+#pragma unroll
+        for (int s = 0; s < KS; ++s)
+#pragma unroll
+            for (int h = 0; h < 2; ++h) up[h] = Mfma<T>::mma(a[c2][s], bv[2 * h + c2][s], up[h]);
+    V4 g0 = V4{0, 0, 0, 0}, g1 = V4{0, 0, 0, 0};
+#pragma unroll
+    for (int s = 0; s < KS; ++s) {
+        g0 = Mfma<T>::mma(a2[0][s], up[0][s % 4], g0);
+        g1 = Mfma<T>::mma(a2[1][s], up[1][s % 4], g1);
+    }
+    const V4 gv = g0 + g1;
+    const int row = 16 * rb + col;
+    if (row < rows) {
+        auto store = [&](T *dst, const V4 &v) __attribute__((always_inline)) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int k = Mfma<T>::row(kr, r);
+                if (k < terms) dst[k * rows + row] = v[r];
+            }
+        };
+        store(pa, up[0]);
+        store(pa + node_len, up[1]);
+        store(gp, gv);
+    }
+}
+
 // Two downward levels in one launch (complex64, 16-row-padded terms): the
 // wave of (node c of level l, row block rb) forms psi_l[c] as mode 1 does,
 // stores it, and then forms both children psi_{l+1}[2c + h] for the same
