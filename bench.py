@@ -273,10 +273,12 @@ def path_roofline(stage_ms, esize=8, peak_tflops=PEAK_FP32_TFLOPS):
     sections x p complex terms): S2M reads x, writes tree levels n, n-1
     and, for p <= 16, n-2 (the first M2M levels run in the same kernel);
     the upward pass reads the rest of the levels down to 3 and writes down
-    to 2; the downward pass reads phi_l and psi_(l-1)
+    to 2, the downward pass reads phi_l and psi_(l-1)
     and writes psi_l for l = 2..n (one parent L2L and three M2L flips per
-    node, one flip at the top); the leaf reads x and psi_n and writes the
-    section sums; the combine and each FFT pass read and write N points."""
+    node, one flip at the top), except that a level formed and consumed
+    inside one launch (the paired tree kernels) is not read back; the leaf
+    reads x and psi_n and writes the section sums; the combine and each FFT
+    pass read and write N points."""
     M = N // P_BLOCKS
     S = P_BLOCKS - 1
     s = M // LEAF
@@ -287,13 +289,29 @@ def path_roofline(stage_ms, esize=8, peak_tflops=PEAK_FP32_TFLOPS):
     m2 = M // m1
     nb = N * esize
     fused = 2 if (p <= 16 and n >= 4) else 1  # M2M levels formed by the S2M kernel
+    paired = p <= 16 and P_BLOCKS >= 32        # tree levels two per launch
+    up_bytes, lu = 0, n - 1 - fused             # the library's launch sequence
+    while paired and lu - 1 >= 2:
+        up_bytes += lvl(lu + 1) + lvl(lu) + lvl(lu - 1)
+        lu -= 2
+    for l in range(lu, 1, -1):
+        up_bytes += lvl(l + 1) + lvl(l)
+    down_bytes, l = 2 * lvl(2), 3               # top flip, then levels 3..n
+    if paired:
+        last = n - 2
+        if (last - 2) % 2 == 1:
+            down_bytes += 2 * lvl(l) + lvl(l - 1)
+            l += 1
+        while l + 1 <= last:
+            down_bytes += lvl(l) + lvl(l - 1) + lvl(l + 1) + lvl(l) + lvl(l + 1)
+            l += 2
+    for ll in range(l, n + 1):
+        down_bytes += 2 * lvl(ll) + lvl(ll - 1)
     count = {
         "s2m+sums": (4.0 * S * M * p + 8.0 * p * p * S * sum(s >> f for f in range(1, fused + 1)),
                      nb + sum(lvl(n - f) for f in range(fused + 1))),
-        "upward": (8.0 * p * p * S * ((1 << (n - fused)) - 4),
-                   sum(lvl(l + 1) + lvl(l) for l in range(2, n - fused))),
-        "downward": (16.0 * p * p * S * sum(1 << l for l in range(3, n + 1)) + 4.0 * p * p * S * 4,
-                     sum(2 * lvl(l) + lvl(l - 1) for l in range(3, n + 1)) + 2 * lvl(2)),
+        "upward": (8.0 * p * p * S * ((1 << (n - fused)) - 4), up_bytes),
+        "downward": (16.0 * p * p * S * sum(1 << l for l in range(3, n + 1)) + 4.0 * p * p * S * 4, down_bytes),
         "leaf": (4.0 * S * M * (3 * LEAF + p), nb + lvl(n) + nb),
         "combine": (5.0 * N * math.log2(P_BLOCKS), 2 * nb),
         "fft1": (5.0 * N * math.log2(m1), 2 * nb),
