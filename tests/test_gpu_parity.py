@@ -208,10 +208,12 @@ def test_in_place_equals_out_of_place(fmm, orc, torch):
     assert torch.equal(out, xd)
 
 
+# (2^18, P = 64): the complex64 plan runs the matrix-core S2M quad form and
+# the paired tree-level kernels, each signal of the batch on grid z
+@pytest.mark.parametrize("n,p,count", [(1 << 15, 8, 5), (1 << 18, 64, 3)])
 @pytest.mark.parametrize("dtype", [0, 1])
-def test_batched_equals_single(fmm, orc, torch, dtype):
-    say(f"batched dtype={dtype}")
-    n, p, count = 1 << 15, 8, 5
+def test_batched_equals_single(fmm, orc, torch, dtype, n, p, count):
+    say(f"batched dtype={dtype} n={n} p={p}")
     xs = np.stack([orc.normal(n, 100 + i) for i in range(count)])
     plan = fmm.Plan(n, p, 16, 32, dtype, max_batch=count)
     xd = to_dev(torch, xs, plan.torch_dtype)
