@@ -111,13 +111,17 @@ constexpr int pick_radix(int rem) {
     return rem == 2 ? 2 : rem == 4 ? 4 : rem == 8 ? 8 : rem == 32 ? 8 : rem == 512 ? 8 : 16;
 }
 
-// LDS index of element e of a row: one pad slot per 16 elements breaks the
-// bank aliasing of the radix-16 stride.
-__device__ inline int lpad(int e) { return e + (e >> 4); }
+// LDS index of element e of a row: one pad slot per 2^PS elements breaks the
+// bank aliasing of the radix-2^PS stride (16 for the local FFTs; 8 for the
+// combine's radix-8 FP64 stages, whose first stage writes at a stride of
+// 8 elements of 16 bytes)
+template <int PS>
+__device__ inline int lpadp(int e) { return e + (e >> PS); }
+__device__ inline int lpad(int e) { return lpadp<4>(e); }
 
 // One Stockham stage (current sub-transform size Ns, radix R) over C rows,
 // NT threads.
-template <typename T, int L, int C, int Ns, int R, int NT>
+template <typename T, int L, int C, int Ns, int R, int NT, int PS = 4>
 __device__ inline void stockham_stage(cx<T> *a, int ls, const cx<T> *__restrict__ twL) {
     constexpr int NB = L / R;
     constexpr int TOT = C * NB;
@@ -132,7 +136,7 @@ __device__ inline void stockham_stage(cx<T> *a, int ls, const cx<T> *__restrict_
             const int k = j & (Ns - 1);
 #pragma unroll
             for (int r = 0; r < R; ++r) {
-                cx<T> x = row[lpad(j + r * NB)];
+                cx<T> x = row[lpadp<PS>(j + r * NB)];
                 if (Ns > 1 && r > 0) x = cmulT(x, twL[r * k * (L / (Ns * R))]);
                 v[bq][r] = x;
             }
@@ -149,23 +153,20 @@ __device__ inline void stockham_stage(cx<T> *a, int ls, const cx<T> *__restrict_
             const int k = j & (Ns - 1);
             const int d = (j / Ns) * Ns * R + k;
 #pragma unroll
-            for (int r = 0; r < R; ++r) row[lpad(d + r * Ns)] = v[bq][r];
+            for (int r = 0; r < R; ++r) row[lpadp<PS>(d + r * Ns)] = v[bq][r];
         }
     }
     __syncthreads();
 }
 
-template <typename T, int L, int C, int Ns, int NT>
+template <typename T, int L, int C, int Ns, int NT, int PS = 4>
 __device__ inline void stockham(cx<T> *a, int ls, const cx<T> *__restrict__ twL) {
     if constexpr (Ns < L) {
         constexpr int R = pick_radix<T>(L / Ns);
-        stockham_stage<T, L, C, Ns, R, NT>(a, ls, twL);
-        stockham<T, L, C, Ns * R, NT>(a, ls, twL);
+        stockham_stage<T, L, C, Ns, R, NT, PS>(a, ls, twL);
+        stockham<T, L, C, Ns * R, NT, PS>(a, ls, twL);
     }
 }
-
-template <int L>
-constexpr int row_stride() { return L + L / 16 + 1; }
 
 // Row stride (in elements) of a tile of C rows of length L holding elements
 // of `esize` bytes.  Inside a row one pad slot per 16 elements (lpad) breaks
