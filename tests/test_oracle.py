@@ -254,3 +254,19 @@ def test_dft_invariants(orc):
     assert rel(Fs, Fa * np.exp(-2j * np.pi * 5 * k / N)) < 1e-12
     # unnormalised inverse
     assert rel(pl.fmmfft(Fa, -1) / N, a) < 1e-13
+
+
+def test_error_metrics_match_numpy(orc):
+    """orc_rel_l2, orc_max_abs and orc_max_rel_pct against their numpy
+    definitions (the error metrics every parity test reports), including a
+    single planted outlier that only the max metrics see."""
+    x = orc.normal(4096, 31)
+    y = x + 1e-7 * orc.normal(4096, 32)
+    y[123] += 0.5 - 0.25j
+    d = x - y
+    assert abs(orc.rel_l2(x, y) - np.linalg.norm(d) / np.linalg.norm(y)) < 1e-15
+    assert orc.max_abs(x, y) == pytest.approx(np.max(np.abs(d)), rel=1e-15)
+    assert orc.max_rel_pct(x, y) == pytest.approx(np.max(np.abs(d) / np.abs(y)) * 100, rel=1e-15)
+    # the operands are not interchangeable: the reference value is the denominator
+    assert orc.rel_l2(x, 2 * x) == pytest.approx(0.5, rel=1e-15)
+    assert orc.rel_l2(x, x) == 0.0 and orc.max_abs(x, x) == 0.0
