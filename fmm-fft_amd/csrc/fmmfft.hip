@@ -138,6 +138,7 @@ struct Shapes {
     int leaf_G, leaf_bb;
     size_t leaf_lds;
     bool leaf_mma;  // matrix-core leaf kernel (leaf size a multiple of 16)
+    bool leaf_h16;  // complex64 leaf 32 / 64: near field as FP16 split products
 };
 
 // Register/LDS padding of the Chebyshev order for the row kernels.
@@ -155,6 +156,13 @@ size_t leaf_lds(const Geom &g, size_t esize, int G, int bb) {
 int leaf_ct(const Geom &g) {
     const int ct = g.b >= 32 ? 1 : 4;
     return std::min(ct, g.s / 8);
+}
+
+size_t leaf_h16_lds(const Geom &g, int G) {
+    const int nrt = g.b / 16;
+    const size_t wstr = nrt == 4 ? (size_t)LeafH16<4>::WSTR : (size_t)LeafH16<2>::WSTR;
+    const size_t tp = 4 * (size_t)leaf_kt(g.terms), lstr = tp * 2 * G + 2;
+    return (2 * G * wstr + (size_t)G * 4 * g.b + 9 * lstr + (size_t)G * 2 * tp + 4 * (size_t)G) * sizeof(float);
 }
 
 size_t leaf_mma_lds(const Geom &g, size_t esize, int G) {
@@ -231,6 +239,8 @@ Shapes shapes(const Geom &g, size_t esize) {
         sh.leaf_G = Gm;
         sh.leaf_bb = 8;
         sh.leaf_lds = leaf_mma_lds(g, esize, Gm);
+        sh.leaf_h16 = esize == 4 && (g.b == 32 || g.b == 64);
+        if (sh.leaf_h16) sh.leaf_lds = leaf_h16_lds(g, Gm);
     }
 
 
@@ -265,6 +275,7 @@ struct fmmfft_plan_s {
     int device;
     double *tab64 = nullptr;  // FP64 tables (TableLayout offsets)
     float *tab32 = nullptr;   // FP32 copies of the operator block + twiddles
+    _Float16 *kfrag = nullptr;  // FP16 split near-field fragments (leaf_h16)
     size_t tw32_A = 0, tw32_B = 0;  // float offsets of FP32 stage twiddles
     void *work = nullptr;     // default workspace
     int64_t max_batch = 1;
@@ -446,6 +457,51 @@ void launch_leaf_mma(const Geom &g, const Shapes &sh, unsigned nz, hipStream_t s
     }
 }
 
+template <int NRT, int G, int KT>
+void leaf_h16_inst(const fmmfft_plan_s *p, unsigned nz, hipStream_t st, const c32 *x, const float *psi,
+                   int64_t wstride_T, c32 *psec, int64_t wstride_cx, const TabPtrs<float> &tb, int conj) {
+    const Geom &g = p->prm.g;
+    const Shapes &sh = p->sh;
+    auto kern = k_leaf_h16<NRT, G, KT>;
+    int dev = 0, cus = 0, per_cu = 0;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (sh.leaf_lds > 64 * 1024)
+        (void)hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh.leaf_lds);
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 256, sh.leaf_lds) != hipSuccess || per_cu < 1)
+        per_cu = 1;
+    // persistent over the tiles of all nz signals; a multiple of the section
+    // groups: each workgroup keeps one group
+    const int64_t ngroups = g.P / G;
+    const int64_t ntot = ngroups * (g.s / LeafH16<NRT>::BB) * (int64_t)nz;
+    int64_t nwg64 = std::min<int64_t>(ntot, (int64_t)std::max(cus, 8) * per_cu);
+    nwg64 = std::max<int64_t>(ngroups, nwg64 / ngroups * ngroups);
+    hipLaunchKernelGGL(kern, dim3((unsigned)nwg64, 1, 1), dim3(256), sh.leaf_lds, st, g, x, g.N, psi, wstride_T,
+                       psec, wstride_cx, tb.evfrag, tb.evalm, tb.evalmh, tb.kv,
+                       reinterpret_cast<const h16x4 *>(p->kfrag), p->tab64 + p->tl.kv, conj, sh.rm, (int)nz);
+}
+
+template <int NRT, int G>
+void leaf_h16_kt(const fmmfft_plan_s *p, unsigned nz, hipStream_t st, const c32 *x, const float *psi,
+                 int64_t wstride_T, c32 *psec, int64_t wstride_cx, const TabPtrs<float> &tb, int conj) {
+    if (leaf_kt(p->prm.g.terms) == 4)
+        leaf_h16_inst<NRT, G, 4>(p, nz, st, x, psi, wstride_T, psec, wstride_cx, tb, conj);
+    else
+        leaf_h16_inst<NRT, G, 8>(p, nz, st, x, psi, wstride_T, psec, wstride_cx, tb, conj);
+}
+
+void launch_leaf_h16(const fmmfft_plan_s *p, unsigned nz, hipStream_t st, const c32 *x, const float *psi,
+                     int64_t wstride_T, c32 *psec, int64_t wstride_cx, const TabPtrs<float> &tb, int conj) {
+    const bool g4 = p->sh.leaf_G >= 4;
+    if (p->prm.g.b == 64) {
+        if (g4) leaf_h16_kt<4, 4>(p, nz, st, x, psi, wstride_T, psec, wstride_cx, tb, conj);
+        else leaf_h16_kt<4, 2>(p, nz, st, x, psi, wstride_T, psec, wstride_cx, tb, conj);
+    } else {
+        if (g4) leaf_h16_kt<2, 4>(p, nz, st, x, psi, wstride_T, psec, wstride_cx, tb, conj);
+        else leaf_h16_kt<2, 2>(p, nz, st, x, psi, wstride_T, psec, wstride_cx, tb, conj);
+    }
+}
+
 // The kernel's discarded stores (section 0, padding terms) go to sumsec,
 // which the section-sum reduction launched after it overwrites.
 template <typename T, int VEC, int MT>
@@ -615,7 +671,15 @@ int run(const fmmfft_plan_s *p, int64_t count, const void *d_in, void *d_out, in
     launch_s2m_tree<T>(padded_terms(g.terms), p, g, sh, nz, st, x, phi, psi, wstride_T, sums_part, wstride_c64,
                        sumsec, tb, conj, mark);
     mark(3);
-    if (sh.leaf_mma) {
+    bool leaf_done = false;
+    if constexpr (sizeof(T) == 4) {
+        if (sh.leaf_h16) {
+            launch_leaf_h16(p, nz, st, x, psi, wstride_T, psec, wstride_cx, tb, conj);
+            leaf_done = true;
+        }
+    }
+    if (leaf_done) {
+    } else if (sh.leaf_mma) {
         launch_leaf_mma<T>(g, sh, nz, st, x, psi, wstride_T, psec, wstride_cx, tb, conj);
     } else {
         hipLaunchKernelGGL(k_leaf<T>, dim3(g.s / sh.leaf_bb, g.P / sh.leaf_G, nz), dim3(256), sh.leaf_lds, st, g,
@@ -692,6 +756,8 @@ int fmmfft_plan_sizes(int64_t n, int32_t p_blocks, int32_t terms, int32_t leaf, 
     if (table_bytes) {
         size_t t = tl.total * sizeof(double);
         if (dtype == FMMFFT_C64) t += (tl.n_ops + 2 * f.M1 + 2 * f.M2) * sizeof(float);
+        if (dtype == FMMFFT_C64 && sh.leaf_h16)
+            t += (size_t)prm.g.S * (3 * prm.g.b / 16 + prm.g.b / 16 - 1) * 2 * 64 * 4 * sizeof(_Float16);
         *table_bytes = t;
     }
     if (work_bytes) *work_bytes = wl.stride;
@@ -780,6 +846,18 @@ int fmmfft_plan_create(fmmfft_plan *out, int64_t n, int32_t p_blocks, int32_t te
             hipLaunchKernelGGL(k_to_f32, dim3(blocks_for(nB)), dim3(256), 0, st, t + tl.twB,
                                p->tab32 + p->tw32_B, (int64_t)nB);
     }
+    if (prm.dtype == FMMFFT_C64 && p->sh.leaf_h16) {
+        const int nrt = g.b / 16, nf = 3 * g.b / 16 + nrt - 1;
+        const int64_t nk = (int64_t)g.S * nf * 2 * 64 * 4;
+        hipError_t e = hipMalloc(&p->kfrag, nk * sizeof(_Float16));
+        if (e != hipSuccess) {
+            (void)hipStreamDestroy(st);
+            g_last_hip_error.store((int)e);
+            return fail(e == hipErrorOutOfMemory ? FMMFFT_ENOMEM : FMMFFT_EHIP);
+        }
+        hipLaunchKernelGGL(k_build_kfrag, dim3(blocks_for(nk)), dim3(256), 0, st, g, (const double *)(t + tl.kv),
+                           p->kfrag, nrt, nf);
+    }
     hipError_t e1 = hipGetLastError();
     hipError_t e2 = hipStreamSynchronize(st);
     (void)hipStreamDestroy(st);
@@ -852,6 +930,7 @@ int fmmfft_plan_destroy(fmmfft_plan plan) {
     prof_free(plan);
     if (plan->tab64) (void)hipFree(plan->tab64);
     if (plan->tab32) (void)hipFree(plan->tab32);
+    if (plan->kfrag) (void)hipFree(plan->kfrag);
     if (plan->work) (void)hipFree(plan->work);
     delete plan;
     return FMMFFT_OK;

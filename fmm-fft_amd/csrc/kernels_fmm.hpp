@@ -905,6 +905,298 @@ void k_leaf_mma(Geom g, const cx<T> *__restrict__ x, int64_t x_stride,
 }
 
 // ---------------------------------------------------------------------------
+// Leaf evaluation for complex64 on the FP16 matrix pipe (leaf 32 or 64).
+//
+// Same sums and tiles as k_leaf_mma (8 boxes x G sections per tile, one
+// section per wave, persistent), with the near field D = K X run as FP16
+// split products on v_mfma_f32_16x16x16_f16 with FP32 accumulation:
+//   K = Kh + Kl (2^14 K_sc split into two FP16 parts at plan time, kfrag),
+//   X = s (Xh + Xl) (the wave's window scaled by a power of two s so that
+//       its largest |value| lies in [2^14, 2^15), split on the fly),
+//   D = (Kh Xh + Kh Xl + Kl Xh) / (2^14 s),
+// each operand carried to ~2^-22 and only the Kl Xl term (~2^-22) dropped.
+// The singular near-diagonal samples d = nu - mu in {0, -1} (|d + sc/P| <
+// 1, the largest |K| by far: up to P/pi) are not in the FP16 product: they
+// are added per target in FP64 from the exact FP32 window and the FP64
+// kernel values.  L2T (K = terms) and the target-0 half-shift correction
+// stay FP32 as in k_leaf_mma (src/fmmfft.c:751-954).
+//
+// A fragments (Toeplitz: A(rt, ks) = F(ks - rt), NF = 3b/16 + NRT - 1 of
+// them) stay in registers for the whole kernel (the wave's section never
+// changes).  The window sits in LDS as FP32, one array per (section,
+// component) with a pad of 8 floats per box, so that the B operand of K
+// step ks (4 consecutive window rows of one box and component per lane) is
+// one conflict-free 16-byte read: box stride 4(b+8) bytes is an odd
+// multiple of 32 modulo 256 and the component arrays differ by 16 modulo
+// 256.  Loads of the next tile are issued into registers before the current
+// tile computes, in units of 4 rows of one section (written to LDS as one
+// 16-byte store per component); the per-section maxima for the scales come
+// from the same registers.
+// ---------------------------------------------------------------------------
+typedef _Float16 h16x4 __attribute__((ext_vector_type(4)));
+
+template <int NRT>
+struct LeafH16 {
+    static constexpr int b = 16 * NRT;
+    static constexpr int BB = 8, NBOX = BB + 3, ROWS = NBOX * b;
+    static constexpr int BSTR = b + 8;                                   // floats per window box
+    static constexpr int WSTR = NBOX * BSTR + ((4 - (NBOX * BSTR) % 64) % 64 + 64) % 64;  // = 4 mod 64
+    static constexpr int NKS = 3 * b / 16;                               // K steps of the near field
+    static constexpr int NF = NKS + NRT - 1;                             // Toeplitz fragments
+};
+
+template <int NRT, int G, int KT>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+void k_leaf_h16(Geom g, const c32 *__restrict__ x, int64_t x_stride, const float *__restrict__ psi,
+                int64_t tree_stride, c32 *__restrict__ psec, int64_t psec_stride,
+                const float *__restrict__ evfrag, const float *__restrict__ evalm,
+                const float *__restrict__ evalmh, const float *__restrict__ kv,
+                const h16x4 *__restrict__ kfrag, const double *__restrict__ kv64, int conj, RowMap rm,
+                int nsig) {
+    using V4 = Mfma<float>::v4;
+    using L = LeafH16<NRT>;
+    constexpr int b = L::b, BB = L::BB, ROWS = L::ROWS, BSTR = L::BSTR, WSTR = L::WSTR;
+    constexpr int NKS = L::NKS, NF = L::NF;
+    constexpr int NU = ROWS / 4 * G, UPER = (NU + 255) / 256;          // 4-row units of the window
+    constexpr int KVN = G * 4 * b;
+    constexpr int TP = 4 * KT;
+    constexpr int NL = TP * 2 * G;
+    constexpr int LSTR = NL + 2;
+    constexpr int LN = (BB + 1) * NL, LPER = (LN + 255) / 256;
+    constexpr int EN = G * 2 * TP;
+    static_assert(256 % G == 0, "a thread's units share one section");
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int P = g.P, terms = g.terms;
+    const int64_t M = g.M;
+    const int rowsS = tree_rows(g);
+    const int lp = __builtin_ctz(P), mmask = (int)M - 1, smask = g.s - 1;
+    float *xw = reinterpret_cast<float *>(smem);  // [2G][WSTR]: (section, re/im) arrays
+    float *kvs = xw + 2 * G * WSTR;                // [G][4b]
+    float *lsx = kvs + KVN;                        // [BB+1][LSTR]
+    float *evs = lsx + (BB + 1) * LSTR;            // [G][2][TP]
+    float *smax = evs + EN;                        // [4 waves][G]
+    psi += level_off(g, g.n);
+
+    const int ngroups = P / G;
+    const int ntiles = ngroups * (g.s / BB);
+    const int ntot = nsig * ntiles;
+    const int nwg = gridDim.x, w = blockIdx.x;
+    const int xo = (nwg % 8 == 0) ? (w % 8) * (nwg / 8) + w / 8 : w;
+    const int sc0 = (xo % ngroups) * G;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int col = lane & 15, kr = lane >> 4;
+    const int bx = col >> 1, c = col & 1;
+    const int gg = wave, sc = sc0 + gg;
+
+    for (int e = threadIdx.x; e < KVN; e += 256) {
+        const int sce = min(max(sc0 + e / (4 * b), 1), P - 1);
+        kvs[e] = kv[(sce - 1) * 4 * b + e % (4 * b)];
+    }
+    for (int e = threadIdx.x; e < EN; e += 256) {
+        const int sce = min(max(sc0 + e / (2 * TP), 1), P - 1), kk = e % TP;
+        const float *row = ((e / TP) & 1) ? evalm + (size_t)(sce - 1) * b * terms
+                                          : evalmh + (size_t)max(sce - P / 2, 0) * terms;
+        const float v = row[min(kk, terms - 1)];
+        evs[e] = kk < terms ? v : 0.f;
+    }
+    // this wave's section constants: L2T fragments (FP32), the FP16 split
+    // near-field fragments, and the two singular kernel values in FP64
+    const int scl = min(max(sc, 1), P - 1);
+    float la[NRT][KT];
+    {
+        const float *ef = evfrag + (((scl - 1) * NRT * KT) << 6) + lane;
+#pragma unroll
+        for (int s2 = 0; s2 < KT; ++s2)
+#pragma unroll
+            for (int rt = 0; rt < NRT; ++rt) la[rt][s2] = ef[(rt * KT + s2) << 6];
+    }
+    h16x4 ah[NF], al[NF];
+    {
+        const h16x4 *kf = kfrag + (size_t)(scl - 1) * NF * 2 * 64 + lane;
+#pragma unroll
+        for (int f = 0; f < NF; ++f) {
+            ah[f] = kf[(2 * f) * 64];
+            al[f] = kf[(2 * f + 1) * 64];
+        }
+    }
+    const double k0 = kv64[(size_t)(scl - 1) * 4 * b + 2 * b];      // K_sc(0)
+    const double k1 = kv64[(size_t)(scl - 1) * 4 * b + 2 * b - 1];  // K_sc(-1)
+
+    float xr[UPER][8];  // 4 rows x (re, im) per unit
+    float lr[LPER];
+    auto prefetch = [&](int tile) __attribute__((always_inline)) {
+        const int sig = tile / ntiles, lid = tile - sig * ntiles;
+        const c32 *xs = x + (int64_t)sig * x_stride;
+        const float *ps = psi + (int64_t)sig * tree_stride;
+        const int i0 = (lid / ngroups) * BB;
+        const int mu_w0 = (i0 - 2) * b;
+#pragma unroll
+        for (int u = 0; u < UPER; ++u) {
+            const int e = (NU % 256 == 0) ? threadIdx.x + 256 * u : min((int)threadIdx.x + 256 * u, NU - 1);
+            const int rq = e / G, ge = e % G;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const int mu = (mu_w0 + 4 * rq + k) & mmask;  // periodic window
+                const c32 v = xs[(mu << lp) + sc0 + ge];
+                xr[u][2 * k] = v.re;
+                xr[u][2 * k + 1] = v.im;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < LPER; ++u) {
+            const int e = (LN % 256 == 0) ? threadIdx.x + 256 * u : min((int)threadIdx.x + 256 * u, LN - 1);
+            const int bx9 = e / NL, rem = e % NL;
+            const int kk = rem / (2 * G), rr = rem % (2 * G);
+            const int sce = sc0 + (rr >> 1);
+            const int box = (i0 - 1 + bx9) & smask;
+            lr[u] = ps[(box * terms + min(kk, terms - 1)) * rowsS + 2 * sce + (rr & 1)];
+        }
+    };
+    // window rows to LDS (imaginary parts negated for the inverse) and the
+    // per-section maximum magnitude of this wave's units to smax
+    auto commit = [&]() __attribute__((always_inline)) {
+        float m = 0.f;
+        const float sg = conj ? -1.f : 1.f;
+#pragma unroll
+        for (int u = 0; u < UPER; ++u) {
+            const int e = threadIdx.x + 256 * u;
+            if (NU % 256 == 0 || e < NU) {
+                const int rq = e / G, ge = e % G;
+                const int q = (4 * rq) / b, j = (4 * rq) % b;
+                float *dre = xw + (2 * ge) * WSTR + q * BSTR + j;
+                *reinterpret_cast<float4 *>(dre) = make_float4(xr[u][0], xr[u][2], xr[u][4], xr[u][6]);
+                *reinterpret_cast<float4 *>(dre + WSTR) =
+                    make_float4(sg * xr[u][1], sg * xr[u][3], sg * xr[u][5], sg * xr[u][7]);
+#pragma unroll
+                for (int k = 0; k < 8; ++k) m = fmaxf(m, fabsf(xr[u][k]));
+            }
+        }
+        // lanes of equal section: lane % G
+#pragma unroll
+        for (int o = G; o < 64; o <<= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
+        if (lane < G) smax[wave * G + lane] = m;
+#pragma unroll
+        for (int u = 0; u < LPER; ++u) {
+            const int e = threadIdx.x + 256 * u;
+            if (LN % 256 == 0 || e < LN) lsx[(e / NL) * LSTR + e % NL] = lr[u];
+        }
+    };
+
+    int k = 0;
+    int tile = xo;
+    if (tile < ntot) prefetch(tile);
+    while (tile < ntot) {
+        __syncthreads();  // the previous tile is no longer read
+        commit();
+        __syncthreads();
+        const int sig = tile / ntiles, lid = tile - sig * ntiles;
+        c32 *psec_s = psec + (int64_t)sig * psec_stride;
+        const int i0 = (lid / ngroups) * BB;
+        const int ntile = (++k) * nwg + xo;
+        prefetch(ntile < ntot ? ntile : tile);
+
+        if (gg < G) {
+            const float *xre = xw + (2 * gg) * WSTR, *xim = xre + WSTR;
+            if (sc == 0) {
+                for (int t = lane; t < BB * b; t += 64) {
+                    const int o = (2 + t / b) * BSTR + t % b;
+                    psec_s[rm(i0 * b + t)] = c32{xre[o], xim[o]};
+                }
+            } else {
+                // block scale of this section's window: max |value| -> [2^14, 2^15)
+                float mx = smax[gg];
+#pragma unroll
+                for (int w2 = 1; w2 < 4; ++w2) mx = fmaxf(mx, smax[w2 * G + gg]);
+                int ex = 0;
+                (void)frexpf(mx, &ex);
+                const int es = mx > 0.f ? min(max(15 - ex, -126), 127) : 0;
+                const float sx = ldexpf(1.f, es), us = ldexpf(1.f, -es - 14);
+                const float *xc = c ? xim : xre;  // this lane's component
+                V4 acc[NRT];
+#pragma unroll
+                for (int rt = 0; rt < NRT; ++rt) acc[rt] = V4{0, 0, 0, 0};
+                // near field: K step ks takes window rows (bx+1) b + 16 ks + 4 kr .. +3
+#pragma unroll
+                for (int ks = 0; ks < NKS; ++ks) {
+                    const int wr = 16 * ks + 4 * kr;  // offset past box bx+1; 4 rows never cross a box
+                    const float4 v = *reinterpret_cast<const float4 *>(xc + (bx + 1 + wr / b) * BSTR + wr % b);
+                    const float xv[4] = {v.x * sx, v.y * sx, v.z * sx, v.w * sx};
+                    h16x4 xh, xl;
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        xh[i] = (_Float16)xv[i];
+                        xl[i] = (_Float16)(xv[i] - (float)xh[i]);
+                    }
+#pragma unroll
+                    for (int rt = 0; rt < NRT; ++rt) {
+                        const int f = ks - rt + NRT - 1;
+                        acc[rt] = __builtin_amdgcn_mfma_f32_16x16x16f16(ah[f], xh, acc[rt], 0, 0, 0);
+                        acc[rt] = __builtin_amdgcn_mfma_f32_16x16x16f16(ah[f], xl, acc[rt], 0, 0, 0);
+                        acc[rt] = __builtin_amdgcn_mfma_f32_16x16x16f16(al[f], xh, acc[rt], 0, 0, 0);
+                    }
+                }
+#pragma unroll
+                for (int rt = 0; rt < NRT; ++rt) acc[rt] *= us;
+                // far field (L2T) in FP32
+#pragma unroll
+                for (int s2 = 0; s2 < KT; ++s2) {
+                    const float bvl = lsx[(bx + 1) * LSTR + 2 * gg + c + (4 * s2 + kr) * 2 * G];
+#pragma unroll
+                    for (int rt = 0; rt < NRT; ++rt) acc[rt] = Mfma<float>::mma(la[rt][s2], bvl, acc[rt]);
+                }
+                // singular samples in FP64: K(0) x[mu] + K(-1) x[mu-1] for the
+                // lane's targets mu = 16 rt + 4 kr + r of box bx + 2
+#pragma unroll
+                for (int rt = 0; rt < NRT; ++rt) {
+                    const int j0 = 16 * rt + 4 * kr;
+                    const float4 v = *reinterpret_cast<const float4 *>(xc + (bx + 2) * BSTR + j0);
+                    const float vp = xc[j0 == 0 ? (bx + 1) * BSTR + b - 1 : (bx + 2) * BSTR + j0 - 1];
+                    const float xs4[5] = {vp, v.x, v.y, v.z, v.w};
+#pragma unroll
+                    for (int r = 0; r < 4; ++r)
+                        acc[rt][r] += (float)(k0 * (double)xs4[r + 1] + k1 * (double)xs4[r]);
+                }
+                if (2 * sc >= P) {
+                    // re-home target 0 of box bx to box bx - 1 (as k_leaf_mma)
+                    const float *kr_ = kvs + gg * 4 * b;
+                    float corr = 0;
+                    const float *xm2 = xc + bx * BSTR;        // box i-2
+                    const float *xp1 = xc + (bx + 3) * BSTR;  // box i+1
+#pragma unroll 4
+                    for (int j = 0; j < b / 4; ++j) {
+                        const int t = 4 * j + kr;
+                        corr += kr_[t] * xm2[t] - kr_[3 * b + t] * xp1[t];
+                    }
+                    const float *lm = lsx + bx * LSTR + 2 * gg + c;        // box i-1
+                    const float *lb = lsx + (bx + 1) * LSTR + 2 * gg + c;  // box i
+                    const float *eh = evs + gg * 2 * TP, *e0 = eh + TP;
+#pragma unroll
+                    for (int j = 0; j < KT; ++j)
+                        corr += eh[4 * j + kr] * lm[(4 * j + kr) * 2 * G] - e0[4 * j + kr] * lb[(4 * j + kr) * 2 * G];
+                    corr += __shfl_xor(corr, 16, 64);
+                    corr += __shfl_xor(corr, 32, 64);
+                    if (kr == 0) acc[0][0] += (2 * sc == P) ? 0.5f * corr : corr;
+                }
+                // write: pair lanes (c = 0, 1) swap halves (as k_leaf_mma)
+#pragma unroll
+                for (int rt = 0; rt < NRT; ++rt) {
+                    c32 *dst = psec_s + (int64_t)sc * M;
+                    const int mu = (i0 + bx) * b + 16 * rt;
+                    const float a0 = acc[rt][0], a1 = acc[rt][1], a2 = acc[rt][2], a3 = acc[rt][3];
+                    const float keep0 = bsel(c, a2, a0), keep1 = bsel(c, a3, a1);
+                    const float recv0 = swap_pair(bsel(c, a0, a2));
+                    const float recv1 = swap_pair(bsel(c, a1, a3));
+                    *reinterpret_cast<float4 *>(dst + rm(mu + Mfma<float>::row(kr, 2 * c))) =
+                        make_float4(bsel(c, recv0, keep0), bsel(c, keep0, recv0), bsel(c, recv1, keep1),
+                                    bsel(c, keep1, recv1));
+                }
+            }
+        }
+        tile = ntile;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Tree passes on the matrix cores, one launch per level:
 //   mode 0, upward, parent level l (src/fmmfft.c:476-515, mpp :174-184):
 //     phi_l[q] = shftfn . phi_{l+1}[2q] + shftfp . phi_{l+1}[2q+1]

@@ -255,4 +255,37 @@ __global__ __launch_bounds__(256) void k_to_f32(const double *src, float *dst, i
     if (i < n) dst[i] = (float)src[i];
 }
 
+// The FP32 residual of an FP64 table: dst[i] = src[i] - (float)src[i]
+// rounded to FP32, so that (float)src + dst carries src to ~2^-48.
+__global__ __launch_bounds__(256) void k_to_f32_lo(const double *src, float *dst, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = (float)(src[i] - (double)(float)src[i]);
+}
+
+// Near-field kernel samples of the FP16 leaf kernel in MFMA fragment order.
+// For section sc, fragment f (Toeplitz offset m = f - (nrt - 1)), part h
+// (0 hi, 1 lo) and lane l = 16 kr + col, element i < 4 holds the FP16 part
+// of 2^14 K_sc(d) with kv index 16 m + 4 kr + i - col + b (d = index - 2b):
+// the A operand of v_mfma_f32_16x16x16_f16 for row tile rt and K step ks
+// with m = ks - rt.  The two singular samples (d = 0 and d = -1, where
+// |d + sc/P| < 1) are zero here: the kernel adds them in FP64.
+// hi = fp16(v), lo = fp16(v - hi), so hi + lo = v to ~2^-22.
+__global__ __launch_bounds__(256) void k_build_kfrag(Geom g, const double *kv, _Float16 *kfrag, int nrt, int nf) {
+    const int64_t n = (int64_t)g.S * nf * 2 * 64 * 4;
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    const int i = (int)(e & 3), lane = (int)((e >> 2) & 63);
+    int64_t r = e >> 8;
+    const int h = (int)(r & 1);
+    r >>= 1;
+    const int f = (int)(r % nf);
+    const int64_t sc1 = r / nf;
+    const int b = g.b, m = f - (nrt - 1);
+    const int idx = 16 * m + 4 * (lane >> 4) + i - (lane & 15) + b;
+    double v = 0.0;
+    if (idx >= 0 && idx < 4 * b && idx != 2 * b && idx != 2 * b - 1) v = ldexp(kv[sc1 * 4 * b + idx], 14);
+    const _Float16 hi = (_Float16)(float)v;
+    kfrag[e] = h == 0 ? hi : (_Float16)(float)(v - (double)(float)hi);
+}
+
 }  // namespace fmmfft
