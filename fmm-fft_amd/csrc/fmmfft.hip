@@ -162,8 +162,12 @@ size_t leaf_h16_lds(const Geom &g, int G) {
     const int nrt = g.b / 16;
     const size_t wstr = nrt == 4 ? (size_t)LeafH16<4>::WSTR : (size_t)LeafH16<2>::WSTR;
     const size_t tp = 4 * (size_t)leaf_kt(g.terms), lstr = tp * 2 * G + 2;
-    return (2 * G * wstr + (size_t)G * 4 * g.b + 9 * lstr + (size_t)G * 2 * tp + 4 * (size_t)G) * sizeof(float);
+    const size_t nwv = G > 4 ? G : 4;
+    return (2 * G * wstr + (size_t)G * 4 * g.b + 9 * lstr + (size_t)G * 2 * tp + nwv * (size_t)G) * sizeof(float);
 }
+
+// sections per tile of the FP16 leaf kernel (one per wave)
+inline int leaf_h16_G(const Geom &g) { return std::min(g.P, 4); }
 
 size_t leaf_mma_lds(const Geom &g, size_t esize, int G) {
     const int bb = 8 * leaf_ct(g);
@@ -240,7 +244,10 @@ Shapes shapes(const Geom &g, size_t esize) {
         sh.leaf_bb = 8;
         sh.leaf_lds = leaf_mma_lds(g, esize, Gm);
         sh.leaf_h16 = esize == 4 && (g.b == 32 || g.b == 64);
-        if (sh.leaf_h16) sh.leaf_lds = leaf_h16_lds(g, Gm);
+        if (sh.leaf_h16) {
+            sh.leaf_G = leaf_h16_G(g);
+            sh.leaf_lds = leaf_h16_lds(g, sh.leaf_G);
+        }
     }
 
 
@@ -468,7 +475,8 @@ void leaf_h16_inst(const fmmfft_plan_s *p, unsigned nz, hipStream_t st, const c3
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
     if (sh.leaf_lds > 64 * 1024)
         (void)hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh.leaf_lds);
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 256, sh.leaf_lds) != hipSuccess || per_cu < 1)
+    constexpr int nth = 64 * (G > 4 ? G : 4);
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, nth, sh.leaf_lds) != hipSuccess || per_cu < 1)
         per_cu = 1;
     // persistent over the tiles of all nz signals; a multiple of the section
     // groups: each workgroup keeps one group
@@ -476,7 +484,7 @@ void leaf_h16_inst(const fmmfft_plan_s *p, unsigned nz, hipStream_t st, const c3
     const int64_t ntot = ngroups * (g.s / LeafH16<NRT>::BB) * (int64_t)nz;
     int64_t nwg64 = std::min<int64_t>(ntot, (int64_t)std::max(cus, 8) * per_cu);
     nwg64 = std::max<int64_t>(ngroups, nwg64 / ngroups * ngroups);
-    hipLaunchKernelGGL(kern, dim3((unsigned)nwg64, 1, 1), dim3(256), sh.leaf_lds, st, g, x, g.N, psi, wstride_T,
+    hipLaunchKernelGGL(kern, dim3((unsigned)nwg64, 1, 1), dim3(nth), sh.leaf_lds, st, g, x, g.N, psi, wstride_T,
                        psec, wstride_cx, tb.evfrag, tb.evalm, tb.evalmh, tb.kv,
                        reinterpret_cast<const h16x4 *>(p->kfrag), p->tab64 + p->tl.kv, conj, sh.rm, (int)nz);
 }
@@ -492,12 +500,14 @@ void leaf_h16_kt(const fmmfft_plan_s *p, unsigned nz, hipStream_t st, const c32 
 
 void launch_leaf_h16(const fmmfft_plan_s *p, unsigned nz, hipStream_t st, const c32 *x, const float *psi,
                      int64_t wstride_T, c32 *psec, int64_t wstride_cx, const TabPtrs<float> &tb, int conj) {
-    const bool g4 = p->sh.leaf_G >= 4;
+    const int G = p->sh.leaf_G;
     if (p->prm.g.b == 64) {
-        if (g4) leaf_h16_kt<4, 4>(p, nz, st, x, psi, wstride_T, psec, wstride_cx, tb, conj);
+        if (G >= 8) leaf_h16_kt<4, 8>(p, nz, st, x, psi, wstride_T, psec, wstride_cx, tb, conj);
+        else if (G >= 4) leaf_h16_kt<4, 4>(p, nz, st, x, psi, wstride_T, psec, wstride_cx, tb, conj);
         else leaf_h16_kt<4, 2>(p, nz, st, x, psi, wstride_T, psec, wstride_cx, tb, conj);
     } else {
-        if (g4) leaf_h16_kt<2, 4>(p, nz, st, x, psi, wstride_T, psec, wstride_cx, tb, conj);
+        if (G >= 8) leaf_h16_kt<2, 8>(p, nz, st, x, psi, wstride_T, psec, wstride_cx, tb, conj);
+        else if (G >= 4) leaf_h16_kt<2, 4>(p, nz, st, x, psi, wstride_T, psec, wstride_cx, tb, conj);
         else leaf_h16_kt<2, 2>(p, nz, st, x, psi, wstride_T, psec, wstride_cx, tb, conj);
     }
 }

@@ -946,7 +946,7 @@ struct LeafH16 {
 };
 
 template <int NRT, int G, int KT>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+__global__ __launch_bounds__(64 * (G > 4 ? G : 4)) __attribute__((amdgpu_waves_per_eu(2)))
 void k_leaf_h16(Geom g, const c32 *__restrict__ x, int64_t x_stride, const float *__restrict__ psi,
                 int64_t tree_stride, c32 *__restrict__ psec, int64_t psec_stride,
                 const float *__restrict__ evfrag, const float *__restrict__ evalm,
@@ -957,14 +957,16 @@ void k_leaf_h16(Geom g, const c32 *__restrict__ x, int64_t x_stride, const float
     using L = LeafH16<NRT>;
     constexpr int b = L::b, BB = L::BB, ROWS = L::ROWS, BSTR = L::BSTR, WSTR = L::WSTR;
     constexpr int NKS = L::NKS, NF = L::NF;
-    constexpr int NU = ROWS / 4 * G, UPER = (NU + 255) / 256;          // 4-row units of the window
+    constexpr int NU = ROWS / 4 * G;                                   // 4-row units of the window
     constexpr int KVN = G * 4 * b;
     constexpr int TP = 4 * KT;
     constexpr int NL = TP * 2 * G;
     constexpr int LSTR = NL + 2;
-    constexpr int LN = (BB + 1) * NL, LPER = (LN + 255) / 256;
+    constexpr int NWV = G > 4 ? G : 4, NTH = 64 * NWV;              // one section per wave
+    constexpr int LN = (BB + 1) * NL, LPER = (LN + NTH - 1) / NTH;
     constexpr int EN = G * 2 * TP;
-    static_assert(256 % G == 0, "a thread's units share one section");
+    constexpr int UPER = (NU + NTH - 1) / NTH;
+    static_assert(NTH % G == 0, "a thread's units share one section");
     extern __shared__ __align__(16) unsigned char smem[];
     const int P = g.P, terms = g.terms;
     const int64_t M = g.M;
@@ -974,7 +976,7 @@ void k_leaf_h16(Geom g, const c32 *__restrict__ x, int64_t x_stride, const float
     float *kvs = xw + 2 * G * WSTR;                // [G][4b]
     float *lsx = kvs + KVN;                        // [BB+1][LSTR]
     float *evs = lsx + (BB + 1) * LSTR;            // [G][2][TP]
-    float *smax = evs + EN;                        // [4 waves][G]
+    float *smax = evs + EN;                        // [NWV waves][G]
     psi += level_off(g, g.n);
 
     const int ngroups = P / G;
@@ -988,11 +990,11 @@ void k_leaf_h16(Geom g, const c32 *__restrict__ x, int64_t x_stride, const float
     const int bx = col >> 1, c = col & 1;
     const int gg = wave, sc = sc0 + gg;
 
-    for (int e = threadIdx.x; e < KVN; e += 256) {
+    for (int e = threadIdx.x; e < KVN; e += NTH) {
         const int sce = min(max(sc0 + e / (4 * b), 1), P - 1);
         kvs[e] = kv[(sce - 1) * 4 * b + e % (4 * b)];
     }
-    for (int e = threadIdx.x; e < EN; e += 256) {
+    for (int e = threadIdx.x; e < EN; e += NTH) {
         const int sce = min(max(sc0 + e / (2 * TP), 1), P - 1), kk = e % TP;
         const float *row = ((e / TP) & 1) ? evalm + (size_t)(sce - 1) * b * terms
                                           : evalmh + (size_t)max(sce - P / 2, 0) * terms;
@@ -1032,7 +1034,7 @@ void k_leaf_h16(Geom g, const c32 *__restrict__ x, int64_t x_stride, const float
         const int mu_w0 = (i0 - 2) * b;
 #pragma unroll
         for (int u = 0; u < UPER; ++u) {
-            const int e = (NU % 256 == 0) ? threadIdx.x + 256 * u : min((int)threadIdx.x + 256 * u, NU - 1);
+            const int e = (NU % NTH == 0) ? threadIdx.x + NTH * u : min((int)threadIdx.x + NTH * u, NU - 1);
             const int rq = e / G, ge = e % G;
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
@@ -1044,7 +1046,7 @@ void k_leaf_h16(Geom g, const c32 *__restrict__ x, int64_t x_stride, const float
         }
 #pragma unroll
         for (int u = 0; u < LPER; ++u) {
-            const int e = (LN % 256 == 0) ? threadIdx.x + 256 * u : min((int)threadIdx.x + 256 * u, LN - 1);
+            const int e = (LN % NTH == 0) ? threadIdx.x + NTH * u : min((int)threadIdx.x + NTH * u, LN - 1);
             const int bx9 = e / NL, rem = e % NL;
             const int kk = rem / (2 * G), rr = rem % (2 * G);
             const int sce = sc0 + (rr >> 1);
@@ -1059,8 +1061,8 @@ void k_leaf_h16(Geom g, const c32 *__restrict__ x, int64_t x_stride, const float
         const float sg = conj ? -1.f : 1.f;
 #pragma unroll
         for (int u = 0; u < UPER; ++u) {
-            const int e = threadIdx.x + 256 * u;
-            if (NU % 256 == 0 || e < NU) {
+            const int e = threadIdx.x + NTH * u;
+            if (NU % NTH == 0 || e < NU) {
                 const int rq = e / G, ge = e % G;
                 const int q = (4 * rq) / b, j = (4 * rq) % b;
                 float *dre = xw + (2 * ge) * WSTR + q * BSTR + j;
@@ -1077,8 +1079,8 @@ void k_leaf_h16(Geom g, const c32 *__restrict__ x, int64_t x_stride, const float
         if (lane < G) smax[wave * G + lane] = m;
 #pragma unroll
         for (int u = 0; u < LPER; ++u) {
-            const int e = threadIdx.x + 256 * u;
-            if (LN % 256 == 0 || e < LN) lsx[(e / NL) * LSTR + e % NL] = lr[u];
+            const int e = threadIdx.x + NTH * u;
+            if (LN % NTH == 0 || e < LN) lsx[(e / NL) * LSTR + e % NL] = lr[u];
         }
     };
 
@@ -1106,7 +1108,7 @@ void k_leaf_h16(Geom g, const c32 *__restrict__ x, int64_t x_stride, const float
                 // block scale of this section's window: max |value| -> [2^14, 2^15)
                 float mx = smax[gg];
 #pragma unroll
-                for (int w2 = 1; w2 < 4; ++w2) mx = fmaxf(mx, smax[w2 * G + gg]);
+                for (int w2 = 1; w2 < NWV; ++w2) mx = fmaxf(mx, smax[w2 * G + gg]);
                 int ex = 0;
                 (void)frexpf(mx, &ex);
                 const int es = mx > 0.f ? min(max(15 - ex, -126), 127) : 0;
@@ -1116,10 +1118,20 @@ void k_leaf_h16(Geom g, const c32 *__restrict__ x, int64_t x_stride, const float
 #pragma unroll
                 for (int rt = 0; rt < NRT; ++rt) acc[rt] = V4{0, 0, 0, 0};
                 // near field: K step ks takes window rows (bx+1) b + 16 ks + 4 kr .. +3
+                // (4 rows never cross a box).  The window rows of step ks + 1
+                // are read before the MFMAs of step ks, and the MFMAs of the
+                // NRT row tiles run interleaved (independent accumulators back
+                // to back; one tile's three products in a row wait on each
+                // other).
+                auto brow = [&](int ks) __attribute__((always_inline)) {
+                    const int wr = 16 * ks + 4 * kr;
+                    return *reinterpret_cast<const float4 *>(xc + (bx + 1 + wr / b) * BSTR + wr % b);
+                };
+                float4 vn = brow(0);
 #pragma unroll
                 for (int ks = 0; ks < NKS; ++ks) {
-                    const int wr = 16 * ks + 4 * kr;  // offset past box bx+1; 4 rows never cross a box
-                    const float4 v = *reinterpret_cast<const float4 *>(xc + (bx + 1 + wr / b) * BSTR + wr % b);
+                    const float4 v = vn;
+                    if (ks + 1 < NKS) vn = brow(ks + 1);
                     const float xv[4] = {v.x * sx, v.y * sx, v.z * sx, v.w * sx};
                     h16x4 xh, xl;
 #pragma unroll
@@ -1128,12 +1140,14 @@ void k_leaf_h16(Geom g, const c32 *__restrict__ x, int64_t x_stride, const float
                         xl[i] = (_Float16)(xv[i] - (float)xh[i]);
                     }
 #pragma unroll
-                    for (int rt = 0; rt < NRT; ++rt) {
-                        const int f = ks - rt + NRT - 1;
-                        acc[rt] = __builtin_amdgcn_mfma_f32_16x16x16f16(ah[f], xh, acc[rt], 0, 0, 0);
-                        acc[rt] = __builtin_amdgcn_mfma_f32_16x16x16f16(ah[f], xl, acc[rt], 0, 0, 0);
-                        acc[rt] = __builtin_amdgcn_mfma_f32_16x16x16f16(al[f], xh, acc[rt], 0, 0, 0);
-                    }
+                    for (int rt = 0; rt < NRT; ++rt)
+                        acc[rt] = __builtin_amdgcn_mfma_f32_16x16x16f16(ah[ks - rt + NRT - 1], xh, acc[rt], 0, 0, 0);
+#pragma unroll
+                    for (int rt = 0; rt < NRT; ++rt)
+                        acc[rt] = __builtin_amdgcn_mfma_f32_16x16x16f16(ah[ks - rt + NRT - 1], xl, acc[rt], 0, 0, 0);
+#pragma unroll
+                    for (int rt = 0; rt < NRT; ++rt)
+                        acc[rt] = __builtin_amdgcn_mfma_f32_16x16x16f16(al[ks - rt + NRT - 1], xh, acc[rt], 0, 0, 0);
                 }
 #pragma unroll
                 for (int rt = 0; rt < NRT; ++rt) acc[rt] *= us;
