@@ -27,7 +27,8 @@ C128 = 1
 FORWARD = 1
 INVERSE = -1
 
-OK, EINVAL, ENOMEM, EHIP, ENOTSUP = 0, 1, 2, 3, 4
+OK, EINVAL, ENOMEM, EHIP, ENOTSUP, ETOOBIG = 0, 1, 2, 3, 4, 5
+STATIC_SCRATCH_BYTES = 64 << 20  # FMMFFT_STATIC_SCRATCH_BYTES
 
 # Symbols declared in include/fmmfft.h.
 EXPORTS = (
@@ -38,6 +39,7 @@ EXPORTS = (
     "fmmfft_execute_batched",
     "fmmfft_workspace_bytes",
     "fmmfft_execute_ws",
+    "fmmfft_execute_static",
     "fmmfft_plan_get",
     "fmmfft_read_table",
     "fmmfft_read_stage",
@@ -89,6 +91,7 @@ def lib():
         L.fmmfft_workspace_bytes.argtypes = [vp, i64]
         L.fmmfft_workspace_bytes.restype = sz
         L.fmmfft_execute_ws.argtypes = [vp, i64, vp, vp, i32, vp, sz, vp]
+        L.fmmfft_execute_static.argtypes = [vp, vp, vp, i32, vp]
         L.fmmfft_plan_get.argtypes = [vp] + [ctypes.c_void_p] * 6
         L.fmmfft_read_table.argtypes = [vp, i32, vp, sz]
         L.fmmfft_read_stage.argtypes = [vp, i32, vp, sz]
@@ -202,6 +205,18 @@ class Plan:
                                        ctypes.c_void_p(work.data_ptr()), work.numel() * work.element_size(),
                                        _stream_ptr(stream)),
                "fmmfft_execute_ws")
+        return out
+
+    def execute_static(self, x, out=None, direction: int = FORWARD, stream=None):
+        """One transform on the library's static device scratch
+        (fmmfft_execute_static: allocates nothing; calls that overlap on one
+        device race)."""
+        self._check_tensor(x, 1)
+        if out is None:
+            out = _torch().empty_like(x)
+        self._check_tensor(out, 1)
+        _check(lib().fmmfft_execute_static(self._h, ctypes.c_void_p(x.data_ptr()), ctypes.c_void_p(out.data_ptr()),
+                                           direction, _stream_ptr(stream)), "fmmfft_execute_static")
         return out
 
     def workspace_bytes(self, count: int = 1) -> int:

@@ -7,6 +7,7 @@
 #include <algorithm>
 #include <atomic>
 #include <cstring>
+#include <mutex>
 #include <new>
 
 #include "common.hpp"
@@ -19,6 +20,33 @@ using namespace fmmfft;
 namespace {
 
 std::atomic<int> g_last_hip_error{0};
+
+// Workspace of fmmfft_execute_static: one array per device, held for the
+// life of the process.  Host code never uses this name as an address (on
+// the host it is only a stand-in): the device address comes from
+// hipGetSymbolAddress, once per device, in static_scratch().
+static __device__ __align__(256) unsigned char scratch[FMMFFT_STATIC_SCRATCH_BYTES];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of the current device's scratch (nullptr on failure; the
+// HIP error is recorded).
+void *static_scratch(int dev) {
+    if (dev < 0 || dev >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr) {
+            g_last_hip_error.store((int)e);
+            return nullptr;
+        }
+        g_scratch_ptr[dev] = ptr;
+    }
+    return g_scratch_ptr[dev];
+}
 
 constexpr size_t kAlign = 256;
 inline size_t align_up(size_t v) { return (v + kAlign - 1) / kAlign * kAlign; }
@@ -974,6 +1002,16 @@ int fmmfft_execute(fmmfft_plan plan, const void *d_in, void *d_out, int32_t dir,
     return fmmfft_execute_batched(plan, 1, d_in, d_out, dir, stream);
 }
 
+int fmmfft_execute_static(fmmfft_plan plan, const void *d_in, void *d_out, int32_t dir, void *stream) {
+    if (!plan) return FMMFFT_EINVAL;
+    if (plan->ws.stride > (size_t)FMMFFT_STATIC_SCRATCH_BYTES) return FMMFFT_ETOOBIG;
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    void *ws = static_scratch(dev);
+    if (!ws) return FMMFFT_EHIP;
+    return fmmfft_execute_ws(plan, 1, d_in, d_out, dir, ws, plan->ws.stride, stream);
+}
+
 int fmmfft_plan_get(fmmfft_plan plan, int64_t *n, int32_t *p_blocks, int32_t *terms, int32_t *leaf,
                     int32_t *dtype, int32_t *depth) {
     if (!plan) return FMMFFT_EINVAL;
@@ -1042,6 +1080,7 @@ const char *fmmfft_strerror(int status) {
         case FMMFFT_ENOMEM: return "out of device memory";
         case FMMFFT_EHIP: return "HIP runtime error";
         case FMMFFT_ENOTSUP: return "parameters not supported by the kernels";
+        case FMMFFT_ETOOBIG: return "workspace larger than the static scratch (FMMFFT_STATIC_SCRATCH_BYTES)";
         default: return "unknown status";
     }
 }

@@ -48,7 +48,9 @@ enum fmmfft_status {
     FMMFFT_EINVAL = 1,  /* invalid (N, P, terms, leaf), dtype, pointer or count */
     FMMFFT_ENOMEM = 2,  /* a device allocation failed                          */
     FMMFFT_EHIP = 3,    /* a HIP runtime call failed (see fmmfft_last_hip_error) */
-    FMMFFT_ENOTSUP = 4  /* valid parameters outside what the kernels support    */
+    FMMFFT_ENOTSUP = 4, /* valid parameters outside what the kernels support    */
+    FMMFFT_ETOOBIG = 5  /* workspace exceeds FMMFFT_STATIC_SCRATCH_BYTES
+                           (fmmfft_execute_static only)                      */
 };
 
 /* Direction: +1 forward X[k] = sum x[n] e^{-2 pi i nk/N}; -1 unnormalised
@@ -92,6 +94,20 @@ int fmmfft_execute(fmmfft_plan plan, const void *d_in, void *d_out, int32_t dir,
  * launch sequence; count <= the plan's max_batch. */
 int fmmfft_execute_batched(fmmfft_plan plan, int64_t count, const void *d_in,
                            void *d_out, int32_t dir, void *stream);
+
+/* One transform that allocates nothing and uses no plan workspace (no
+ * counterpart in the reference: for callers that cannot allocate device
+ * memory -- code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  The workspace is a static device array of
+ * FMMFFT_STATIC_SCRATCH_BYTES, one per device, held until the process ends;
+ * plans whose one-signal workspace (fmmfft_plan_sizes' work_bytes) exceeds it
+ * are rejected with FMMFFT_ETOOBIG.  Every call on a device shares that
+ * device's array: calls that overlap on one device (different streams or
+ * threads without ordering between them) race.  Nothing in the array needs
+ * clearing: each execute writes every word it reads before reading it.
+ * Results equal fmmfft_execute's bit for bit. */
+#define FMMFFT_STATIC_SCRATCH_BYTES (64u << 20)
+int fmmfft_execute_static(fmmfft_plan plan, const void *d_in, void *d_out, int32_t dir, void *stream);
 
 /* Bytes of caller workspace for count signals, and the execute that uses
  * it (d_work must be 256-byte aligned device memory of that size). */
