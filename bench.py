@@ -29,6 +29,7 @@ REPO = os.path.dirname(os.path.abspath(__file__))
 
 N, P_BLOCKS, TERMS, LEAF = 1 << 26, 64, 16, 64
 PEAK_FP32_TFLOPS = 157.3   # MI355X dense FP32 matrix
+PEAK_FP16_TFLOPS = 2500.0  # MI355X dense FP16 matrix
 PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
 
@@ -268,6 +269,23 @@ def measured_traffic(stage):
     return None
 
 
+def leaf_compute_ms():
+    """Matrix-pipe time of the complex64 leaf at peak: near field as three
+    FP16 split products (Kh Xh + Kh Xl + Kl Xh) of 4 S M 3b flops each on
+    the FP16 pipe, L2T (4 S M p flops) on the FP32 pipe."""
+    M, S = N // P_BLOCKS, P_BLOCKS - 1
+    return (3 * 4.0 * S * M * 3 * LEAF / (PEAK_FP16_TFLOPS * 1e12) +
+            4.0 * S * M * TERMS / (PEAK_FP32_TFLOPS * 1e12)) * 1e3
+
+
+def leaf_bytes():
+    """Algorithmic HBM bytes of one complex64 leaf launch: x and the finest
+    local expansions read, the section cot sums written."""
+    M, S = N // P_BLOCKS, P_BLOCKS - 1
+    s = M // LEAF
+    return 2 * N * 8 + s * S * TERMS * 8
+
+
 def path_roofline(stage_ms, esize=8, peak_tflops=PEAK_FP32_TFLOPS):
     """Roofline time of the whole path at this workload: the sum over stages
     of max(algorithmic flops / matrix peak, algorithmic bytes / HBM peak),
@@ -281,7 +299,9 @@ def path_roofline(stage_ms, esize=8, peak_tflops=PEAK_FP32_TFLOPS):
     node, one flip at the top), except that a level formed and consumed
     inside one launch (the paired tree kernels) is not read back; the leaf
     reads x and psi_n and writes the section sums; the combine and each FFT
-    pass read and write N points."""
+    pass read and write N points.  The complex64 leaf runs its near field
+    (4 S M 3b flops) as three FP16 split products on the FP16 matrix pipe
+    and L2T (4 S M p) on the FP32 one: its compute time is priced that way."""
     M = N // P_BLOCKS
     S = P_BLOCKS - 1
     s = M // LEAF
@@ -315,16 +335,17 @@ def path_roofline(stage_ms, esize=8, peak_tflops=PEAK_FP32_TFLOPS):
                      nb + sum(lvl(n - f) for f in range(fused + 1))),
         "upward": (8.0 * p * p * S * ((1 << (n - fused)) - 4), up_bytes),
         "downward": (16.0 * p * p * S * sum(1 << l for l in range(3, n + 1)) + 4.0 * p * p * S * 4, down_bytes),
-        "leaf": (4.0 * S * M * (3 * LEAF + p), nb + lvl(n) + nb),
+        "leaf": (4.0 * S * M * (3 * LEAF + p), nb + lvl(n) + nb, leaf_compute_ms() if esize == 8 else None),
         "combine": (5.0 * N * math.log2(P_BLOCKS), 2 * nb),
         "fft1": (5.0 * N * math.log2(m1), 2 * nb),
         "fft2": (5.0 * N * math.log2(m2), 2 * nb),
     }
     phases, t_roof, t_meas = {}, 0.0, 0.0
-    for st, (fl, by) in count.items():
+    for st, cnt in count.items():
         if st not in stage_ms:
             continue
-        tf = fl / (peak_tflops * 1e12) * 1e3
+        fl, by = cnt[0], cnt[1]
+        tf = cnt[2] if len(cnt) > 2 and cnt[2] is not None else fl / (peak_tflops * 1e12) * 1e3
         tb = by / (PEAK_HBM_GBS * 1e9) * 1e3
         t = max(tf, tb)
         t_roof += t
@@ -390,6 +411,17 @@ def main():
             "rel_l2_fp32_vs_fp64": rel,
         })
         del plan64, xin64, out64, d
+        # the split-product M2L form (FMMFFT_C64_SPLIT) on the same input
+        plans = fmm.Plan(N, P_BLOCKS, TERMS, LEAF, fmm.C64_SPLIT)
+        outs = torch.empty_like(x32)
+        mss, _, stagess = time_plan(torch, plans, x32, outs, steps64, 2, True)
+        plan64 = fmm.Plan(N, P_BLOCKS, TERMS, LEAF, fmm.C128)
+        ref64 = plan64.execute(x32.to(torch.complex128))
+        rels = float(torch.linalg.vector_norm(outs.to(torch.complex128) - ref64) / torch.linalg.vector_norm(ref64))
+        extra["split_m2l_form"] = {"value": round(5 * N * math.log2(N) / (mss * 1e-3) / 1e9, 1), "unit": "GFLOP/s",
+                                   "ms_per_step": round(mss, 4), "rel_l2_vs_fp64": rels,
+                                   "downward_ms": round(stagess[0]["downward"] / steps64, 4)}
+        del plans, outs, plan64, ref64
 
     if not args.no_other_configs:
         extra["other_configs"] = other_configs(torch, fmm)
@@ -400,11 +432,14 @@ def main():
     M = N // P_BLOCKS
     S = P_BLOCKS - 1
     if dom == "leaf":
-        work = 4.0 * S * M * (3 * LEAF + TERMS)  # complex x real MACs of near field + L2T
-        achieved = work / (per_stage[dom] * 1e-3) / 1e12
-        roof = {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": traffic, "kernel": "k_leaf_mma<float>",
-                "algorithmic": f"{work:.4g} flop per launch"}
+        # the FP16 split-product leaf: its matrix-pipe time at peak
+        # (leaf_compute_ms) is below its HBM time, so HBM bounds it
+        by = leaf_bytes()
+        achieved = by / (per_stage[dom] * 1e-3) / 1e9
+        roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic, "kernel": "k_leaf_h16<4,4,4>",
+                "algorithmic": f"{by:.4g} bytes per launch (x and psi_n read, psec written); matrix-pipe time "
+                               f"at peak {leaf_compute_ms():.4f} ms"}
     else:
         by = 2.0 * N * 8  # one read + one write of the complex64 signal
         achieved = by / (per_stage[dom] * 1e-3) / 1e9

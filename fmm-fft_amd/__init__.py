@@ -24,6 +24,7 @@ LIB_PATH = os.path.join(_HERE, "libfmmfft_amd.so")
 
 C64 = 0
 C128 = 1
+C64_SPLIT = 2  # complex64 with the M2L flips as FP32 hi + lo split products
 FORWARD = 1
 INVERSE = -1
 
@@ -162,7 +163,7 @@ class Plan:
     @property
     def torch_dtype(self):
         torch = _torch()
-        return torch.complex64 if self.dtype == C64 else torch.complex128
+        return torch.complex128 if self.dtype == C128 else torch.complex64
 
     def close(self):
         if getattr(self, "_h", None):

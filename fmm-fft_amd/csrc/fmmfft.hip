@@ -73,8 +73,11 @@ struct Params {
     size_t esize;  // bytes of one real in the plan's precision
 };
 
+// complex64 storage (plain, or with split-product M2L flips)
+inline bool is_c64(int dtype) { return dtype == FMMFFT_C64 || dtype == FMMFFT_C64_SPLIT; }
+
 int validate(int64_t n, int32_t P, int32_t terms, int32_t leaf, int32_t dtype, Params *out) {
-    if (dtype != FMMFFT_C64 && dtype != FMMFFT_C128) return FMMFFT_EINVAL;
+    if (!is_c64(dtype) && dtype != FMMFFT_C128) return FMMFFT_EINVAL;
     if (P < 2 || P > 64 || !is_pow2(P) || n <= 0 || n % P) return FMMFFT_EINVAL;
     if (terms < 2 || terms > 32 || leaf < 2 || leaf > 128) return FMMFFT_EINVAL;
     const int64_t M = n / P;
@@ -82,7 +85,7 @@ int validate(int64_t n, int32_t P, int32_t terms, int32_t leaf, int32_t dtype, P
     const int64_t s = M / leaf;
     if (!is_pow2(s) || s < 8) return FMMFFT_EINVAL;
     if (leaf % 4) return FMMFFT_ENOTSUP;                   // leaf lanes own 4 targets
-    if ((int64_t)leaf * P > (dtype == FMMFFT_C64 ? 8192 : 4096)) return FMMFFT_ENOTSUP;
+    if ((int64_t)leaf * P > (is_c64(dtype) ? 8192 : 4096)) return FMMFFT_ENOTSUP;
     if (!is_pow2(M) || M > (int64_t(1) << 20)) return FMMFFT_ENOTSUP;  // two FFT passes of <= 1024
     Geom &g = out->g;
     g.N = n;
@@ -94,7 +97,7 @@ int validate(int64_t n, int32_t P, int32_t terms, int32_t leaf, int32_t dtype, P
     g.s = (int)s;
     g.n = ilog2(s);
     out->dtype = dtype;
-    out->esize = dtype == FMMFFT_C64 ? 4 : 8;
+    out->esize = is_c64(dtype) ? 4 : 8;
     return FMMFFT_OK;
 }
 
@@ -312,6 +315,7 @@ struct fmmfft_plan_s {
     float *tab32 = nullptr;   // FP32 copies of the operator block + twiddles
     _Float16 *kfrag = nullptr;  // FP16 split near-field fragments (leaf_h16)
     size_t tw32_A = 0, tw32_B = 0;  // float offsets of FP32 stage twiddles
+    size_t opf_lo32 = 0;            // float offset of the FP32 residual operator fragments
     void *work = nullptr;     // default workspace
     int64_t max_batch = 1;
     // profiling: FMMFFT_NSTAGES + 1 events per recorded execute
@@ -325,6 +329,7 @@ namespace {
 template <typename T>
 struct TabPtrs {
     const T *initch, *ops, *topflp, *evalm, *evalmh, *kv, *evfrag, *opfrag, *twA, *twB;
+    const T *opfrag_lo;  // FP32 residual of the fragment-ordered operators (complex64), else null
 };
 
 template <typename T>
@@ -334,13 +339,14 @@ template <>
 TabPtrs<double> tab_ptrs<double>(const fmmfft_plan_s *p) {
     const double *t = p->tab64;
     return {t + p->tl.initch, t + p->tl.ops, t + p->tl.topflp, t + p->tl.evalm, t + p->tl.evalmh,
-            t + p->tl.kv,     t + p->tl.evfrag, t + p->tl.opfrag, t + p->tl.twA, t + p->tl.twB};
+            t + p->tl.kv,     t + p->tl.evfrag, t + p->tl.opfrag, t + p->tl.twA, t + p->tl.twB, nullptr};
 }
 template <>
 TabPtrs<float> tab_ptrs<float>(const fmmfft_plan_s *p) {
     const float *t = p->tab32;
     return {t + p->tl.initch, t + p->tl.ops, t + p->tl.topflp, t + p->tl.evalm, t + p->tl.evalmh,
-            t + p->tl.kv,     t + p->tl.evfrag, t + p->tl.opfrag, t + p->tw32_A, t + p->tw32_B};
+            t + p->tl.kv,     t + p->tl.evfrag, t + p->tl.opfrag, t + p->tw32_A, t + p->tw32_B,
+            p->prm.dtype == FMMFFT_C64_SPLIT ? t + p->opf_lo32 : nullptr};
 }
 
 inline unsigned blocks_for(int64_t n, int per = 256) { return (unsigned)((n + per - 1) / per); }
@@ -628,7 +634,18 @@ void s2m_tree_pt(const Geom &g, const Shapes &sh, unsigned nz, hipStream_t st, c
     };
     auto level = [&](auto kern, auto pkern, int l, int mode) {
         hipLaunchKernelGGL(pack ? pkern : kern, grid(l, mode), dim3(256), 0, st, g, l, (const T *)phi, phi,
-                           (const T *)psi, psi, wstride_T, tb.opfrag);
+                           (const T *)psi, psi, wstride_T, tb.opfrag, tb.opfrag_lo);
+    };
+    // downward levels: the split-product M2L instances for FMMFFT_C64_SPLIT
+    const bool split = tb.opfrag_lo != nullptr;
+    auto down = [&](int l, int mode) {
+        if (mode == 2) {
+            if (split) level(k_tree_mma<T, TP, 2, false, true>, k_tree_mma<T, TP, 2, true, true>, l, 2);
+            else level(k_tree_mma<T, TP, 2, false>, k_tree_mma<T, TP, 2, true>, l, 2);
+        } else {
+            if (split) level(k_tree_mma<T, TP, 1, false, true>, k_tree_mma<T, TP, 1, true, true>, l, 1);
+            else level(k_tree_mma<T, TP, 1, false>, k_tree_mma<T, TP, 1, true>, l, 1);
+        }
     };
     // the matrix-core S2M has already formed levels n-1 (and n-2)
     int lu = g.n - 1 - (sh.s2m_vec ? sh.s2m_levels : 0);
@@ -642,7 +659,7 @@ void s2m_tree_pt(const Geom &g, const Shapes &sh, unsigned nz, hipStream_t st, c
     }
     for (; lu >= 2; --lu) level(k_tree_mma<T, TP, 0, false>, k_tree_mma<T, TP, 0, true>, lu, 0);
     mark(2);
-    level(k_tree_mma<T, TP, 2, false>, k_tree_mma<T, TP, 2, true>, 2, 2);
+    down(2, 2);
     int l = 3;
     if constexpr (sizeof(T) == 4 && TP == 16) {
         // complex64, 16 terms, nodes of more than two row blocks: levels
@@ -651,13 +668,19 @@ void s2m_tree_pt(const Geom &g, const Shapes &sh, unsigned nz, hipStream_t st, c
         // faster than as a pair (0.163 vs 0.181 ms at N = 2^26)
         if (!pack) {
             const int last = g.n - 2;
-            if ((last - 2) % 2 == 1) level(k_tree_mma<T, TP, 1, false>, k_tree_mma<T, TP, 1, true>, l++, 1);
-            for (; l + 1 <= last; l += 2)
-                hipLaunchKernelGGL((k_tree_down2<T, TP>), dim3((unsigned)(((int64_t(1) << l) * nrb + 3) / 4), 1, nz),
-                                   dim3(256), 0, st, g, l, (const T *)phi, psi, wstride_T, tb.opfrag);
+            if ((last - 2) % 2 == 1) down(l++, 1);
+            for (; l + 1 <= last; l += 2) {
+                const dim3 gd((unsigned)(((int64_t(1) << l) * nrb + 3) / 4), 1, nz);
+                if (split)
+                    hipLaunchKernelGGL((k_tree_down2<T, TP, true>), gd, dim3(256), 0, st, g, l, (const T *)phi, psi,
+                                       wstride_T, tb.opfrag, tb.opfrag_lo);
+                else
+                    hipLaunchKernelGGL((k_tree_down2<T, TP, false>), gd, dim3(256), 0, st, g, l, (const T *)phi, psi,
+                                       wstride_T, tb.opfrag, tb.opfrag_lo);
+            }
         }
     }
-    for (; l <= g.n; ++l) level(k_tree_mma<T, TP, 1, false>, k_tree_mma<T, TP, 1, true>, l, 1);
+    for (; l <= g.n; ++l) down(l, 1);
 }
 
 template <typename T, typename Mark>
@@ -793,8 +816,10 @@ int fmmfft_plan_sizes(int64_t n, int32_t p_blocks, int32_t terms, int32_t leaf, 
     const WsLayout wl = ws_layout(prm.g, prm.esize, sh);
     if (table_bytes) {
         size_t t = tl.total * sizeof(double);
-        if (dtype == FMMFFT_C64) t += (tl.n_ops + 2 * f.M1 + 2 * f.M2) * sizeof(float);
-        if (dtype == FMMFFT_C64 && sh.leaf_h16)
+        const size_t tp = 4 * (size_t)leaf_kt(prm.g.terms), nopf = ((size_t)(prm.g.n - 2) * NOPS + 1) * tp * tp;
+        if (is_c64(dtype)) t += (tl.n_ops + 2 * f.M1 + 2 * f.M2) * sizeof(float);
+        if (dtype == FMMFFT_C64_SPLIT) t += nopf * sizeof(float);
+        if (is_c64(dtype) && sh.leaf_h16)
             t += (size_t)prm.g.S * (3 * prm.g.b / 16 + prm.g.b / 16 - 1) * 2 * 64 * 4 * sizeof(_Float16);
         *table_bytes = t;
     }
@@ -866,11 +891,14 @@ int fmmfft_plan_create(fmmfft_plan *out, int64_t n, int32_t p_blocks, int32_t te
         hipLaunchKernelGGL(k_build_twiddles, dim3(blocks_for(MB)), dim3(256), 0, st, MB, g.M, int64_t(0),
                            reinterpret_cast<double2 *>(t + tl.twB), reinterpret_cast<double2 *>(t + tl.tlo),
                            reinterpret_cast<double2 *>(t + tl.thi));
-    if (prm.dtype == FMMFFT_C64) {
+    if (is_c64(prm.dtype)) {
         const size_t nA = 2 * (size_t)MA, nB = 2 * (size_t)MB;
+        const size_t tp = 4 * (size_t)leaf_kt(g.terms);
+        const size_t nopf = prm.dtype == FMMFFT_C64_SPLIT ? ((size_t)(g.n - 2) * NOPS + 1) * tp * tp : 0;
         p->tw32_A = tl.n_ops;
         p->tw32_B = tl.n_ops + nA;
-        const size_t n32 = tl.n_ops + nA + nB;
+        p->opf_lo32 = tl.n_ops + nA + nB;
+        const size_t n32 = tl.n_ops + nA + nB + nopf;
         hipError_t e = hipMalloc(&p->tab32, n32 * sizeof(float));
         if (e != hipSuccess) {
             (void)hipStreamDestroy(st);
@@ -883,8 +911,11 @@ int fmmfft_plan_create(fmmfft_plan *out, int64_t n, int32_t p_blocks, int32_t te
         if (p->fft.two_pass)
             hipLaunchKernelGGL(k_to_f32, dim3(blocks_for(nB)), dim3(256), 0, st, t + tl.twB,
                                p->tab32 + p->tw32_B, (int64_t)nB);
+        if (nopf)
+            hipLaunchKernelGGL(k_to_f32_lo, dim3(blocks_for(nopf)), dim3(256), 0, st, t + tl.opfrag,
+                               p->tab32 + p->opf_lo32, (int64_t)nopf);
     }
-    if (prm.dtype == FMMFFT_C64 && p->sh.leaf_h16) {
+    if (is_c64(prm.dtype) && p->sh.leaf_h16) {
         const int nrt = g.b / 16, nf = 3 * g.b / 16 + nrt - 1;
         const int64_t nk = (int64_t)g.S * nf * 2 * 64 * 4;
         hipError_t e = hipMalloc(&p->kfrag, nk * sizeof(_Float16));
@@ -987,7 +1018,7 @@ int fmmfft_execute_ws(fmmfft_plan plan, int64_t count, const void *d_in, void *d
     // the S2M kernel streams the input with 16-byte loads
     if (((uintptr_t)d_in % 16) != 0 || ((uintptr_t)d_out % (2 * plan->prm.esize)) != 0) return FMMFFT_EINVAL;
     hipStream_t st = static_cast<hipStream_t>(stream);
-    if (plan->prm.dtype == FMMFFT_C64) return run<float>(plan, count, d_in, d_out, dir, (char *)d_work, st);
+    if (is_c64(plan->prm.dtype)) return run<float>(plan, count, d_in, d_out, dir, (char *)d_work, st);
     return run<double>(plan, count, d_in, d_out, dir, (char *)d_work, st);
 }
 

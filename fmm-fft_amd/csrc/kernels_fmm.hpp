@@ -1255,16 +1255,21 @@ __host__ __device__ constexpr int tree_mma_rb(int mode) {
 // i.e. P <= 16): a wave takes RB / nrb nodes of one parity (c = 2 RB/nrb
 // (w/2) + w%2 + 2k) with all their row blocks, so small nodes do not cost a
 // wave (and a memory round trip) each.
-template <typename T, int TP, int MODE, bool PACK = false>
+template <typename T, int TP, int MODE, bool PACK = false, bool SPLIT_M2L = false>
 __global__ __launch_bounds__(256) void k_tree_mma(Geom g, int l, const T *__restrict__ phi_in,
                                                   T *__restrict__ phi_out, const T *__restrict__ psi_in,
                                                   T *__restrict__ psi_out, int64_t tree_stride,
-                                                  const T *__restrict__ opfrag) {
+                                                  const T *__restrict__ opfrag, const T *__restrict__ opfrag_lo) {
     using V4 = typename Mfma<T>::v4;
     constexpr int KS = TP / 4, KO = TP / 16;
     constexpr int NT = MODE == 0 ? 2 : MODE == 1 ? 4 : 1;  // sources per node
+    // SPLIT_M2L (complex64): the M2L flips (every source of the top flip,
+    // sources 1..3 of a downward node) as split products: the FP64 operator
+    // is carried as FP32 hi + lo (opfrag_lo = A - fp32(A)), both applied
     constexpr int RB = PACK ? tree_mma_rb_packed<T, TP>(MODE) : tree_mma_rb<T, TP>(MODE);
     constexpr int FR = TP * TP;                             // one operator in fragment order
+    constexpr bool SPLIT = SPLIT_M2L && sizeof(T) == 4 && MODE >= 1;
+    constexpr int T0 = MODE == 1 ? 1 : 0;                   // first split source
     const int terms = g.terms, rows = tree_rows(g);
     const int node_len = rows * terms;
     const int nn = 1 << l, nrb = (rows + 15) >> 4, nw = (nrb + RB - 1) / RB;
@@ -1327,13 +1332,17 @@ __global__ __launch_bounds__(256) void k_tree_mma(Geom g, int l, const T *__rest
     // then the RB row blocks' source fragments at clamped indices.  No
     // zeroing is needed: rows past the end only feed output columns that are
     // not stored, and terms past the end meet zero operator columns.
-    T a[NT][KO][KS], bv[RB][NT][KS];
+    T a[NT][KO][KS], bv[RB][NT][KS], alo[SPLIT ? NT : 1][KO][KS];
 #pragma unroll
     for (int t = 0; t < NT; ++t)
 #pragma unroll
         for (int s = 0; s < KS; ++s)
 #pragma unroll
-            for (int ko = 0; ko < KO; ++ko) a[t][ko][s] = mats[t][(s * KO + ko) * 64 + lane];
+            for (int ko = 0; ko < KO; ++ko) {
+                a[t][ko][s] = mats[t][(s * KO + ko) * 64 + lane];
+                if constexpr (SPLIT)
+                    if (t >= T0) alo[t][ko][s] = (mats[t] - opfrag + opfrag_lo)[(s * KO + ko) * 64 + lane];
+            }
 #pragma unroll
     for (int j = 0; j < RB; ++j) {
         int cj, rbj;
@@ -1362,8 +1371,11 @@ __global__ __launch_bounds__(256) void k_tree_mma(Geom g, int l, const T *__rest
 #pragma unroll
                 for (int j = 0; j < RB; ++j)
 #pragma unroll
-                    for (int ko = 0; ko < KO; ++ko)
+                    for (int ko = 0; ko < KO; ++ko) {
                         acc[j][ko] = Mfma<T>::mma(a[t][ko][s], bv[j][t][s], acc[j][ko]);
+                        if constexpr (SPLIT)
+                            if (t >= T0) acc[j][ko] = Mfma<T>::mma(alo[t][ko][s], bv[j][t][s], acc[j][ko]);
+                    }
 #pragma unroll
         for (int j = 0; j < RB; ++j) {
             int cj, rbj;
@@ -1399,7 +1411,11 @@ __global__ __launch_bounds__(256) void k_tree_mma(Geom g, int l, const T *__rest
 #pragma unroll
                 for (int s = 0; s < KS; ++s)
 #pragma unroll
-                    for (int ko = 0; ko < KO; ++ko) acc[ko] = Mfma<T>::mma(a[t][ko][s], bv[j][t][s], acc[ko]);
+                    for (int ko = 0; ko < KO; ++ko) {
+                        acc[ko] = Mfma<T>::mma(a[t][ko][s], bv[j][t][s], acc[ko]);
+                        if constexpr (SPLIT)
+                            if (t >= T0) acc[ko] = Mfma<T>::mma(alo[t][ko][s], bv[j][t][s], acc[ko]);
+                    }
             if (row < rows) {
 #pragma unroll
                 for (int ko = 0; ko < KO; ++ko)
@@ -1490,9 +1506,10 @@ __global__ __launch_bounds__(256) void k_tree_up2(Geom g, int l, T *__restrict__
 // operands in the D layout's K order, the operator fragments read in that
 // order) and their three flips read phi_{l+1} as usual.  Saves the launch
 // and the re-read of psi_l of the per-level form (src/fmmfft.c:644-749).
-template <typename T, int TP>
+template <typename T, int TP, bool SPLIT = false>
 __global__ __launch_bounds__(256) void k_tree_down2(Geom g, int l, const T *__restrict__ phi, T *__restrict__ psi,
-                                                    int64_t tree_stride, const T *__restrict__ opfrag) {
+                                                    int64_t tree_stride, const T *__restrict__ opfrag,
+                                                    const T *__restrict__ opfrag_lo) {
     using V4 = typename Mfma<T>::v4;
     static_assert(TP == 16, "one 16-term block");
     constexpr int KS = TP / 4, FR = TP * TP;
@@ -1525,8 +1542,9 @@ __global__ __launch_bounds__(256) void k_tree_down2(Geom g, int l, const T *__re
     T *psl = psi + ts + level_off(g, l), *psc = psi + ts + level_off(g, l + 1);
     const T *psp = psi + ts + level_off(g, l - 1) + (int64_t)(c >> 1) * node_len;
 
-    // every load of the wave at once, unconditional at clamped indices
-    T a[4][KS], bv[4][KS], al[2][KS], af[2][3][KS], bc[2][3][KS];
+    // every load of the wave at once, unconditional at clamped indices; with
+    // SPLIT the flips (M2L) carry their FP32 residual operators too
+    T a[4][KS], bv[4][KS], al[2][KS], af[2][3][KS], bc[2][3][KS], alo[4][KS], aflo[2][3][KS];
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
         const T *m = op(l, odd, t);
@@ -1534,6 +1552,7 @@ __global__ __launch_bounds__(256) void k_tree_down2(Geom g, int l, const T *__re
 #pragma unroll
         for (int s = 0; s < KS; ++s) {
             a[t][s] = m[s * 64 + lane];
+            if constexpr (SPLIT) alo[t][s] = t == 0 ? T(0) : (m - opfrag + opfrag_lo)[s * 64 + lane];
             bv[t][s] = src[min(4 * s + kr, terms - 1) * rows + rc];
         }
     }
@@ -1553,6 +1572,7 @@ __global__ __launch_bounds__(256) void k_tree_down2(Geom g, int l, const T *__re
 #pragma unroll
             for (int s = 0; s < KS; ++s) {
                 af[h][t - 1][s] = m[s * 64 + lane];
+                if constexpr (SPLIT) aflo[h][t - 1][s] = (m - opfrag + opfrag_lo)[s * 64 + lane];
                 bc[h][t - 1][s] = src[min(4 * s + kr, terms - 1) * rows + rc];
             }
         }
@@ -1568,9 +1588,16 @@ __global__ __launch_bounds__(256) void k_tree_down2(Geom g, int l, const T *__re
             if (t < 2) {
                 p0 = Mfma<T>::mma(a[2 * t][s], bv[2 * t][s], p0);
                 p1 = Mfma<T>::mma(a[2 * t + 1][s], bv[2 * t + 1][s], p1);
+                if constexpr (SPLIT) {
+                    if (t == 1) p0 = Mfma<T>::mma(alo[2][s], bv[2][s], p0);
+                    p1 = Mfma<T>::mma(alo[2 * t + 1][s], bv[2 * t + 1][s], p1);
+                }
             }
 #pragma unroll
-            for (int h = 0; h < 2; ++h) ch[h] = Mfma<T>::mma(af[h][t][s], bc[h][t][s], ch[h]);
+            for (int h = 0; h < 2; ++h) {
+                ch[h] = Mfma<T>::mma(af[h][t][s], bc[h][t][s], ch[h]);
+                if constexpr (SPLIT) ch[h] = Mfma<T>::mma(aflo[h][t][s], bc[h][t][s], ch[h]);
+            }
         }
     const V4 pl = p0 + p1;
 #pragma unroll

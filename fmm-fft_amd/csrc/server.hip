@@ -113,7 +113,7 @@ namespace {
 size_t shape_bytes(const Key &k, int maxb, bool gathered, size_t *ws_bytes) {
     size_t tb = 0, wb = 0;
     if (fmmfft_plan_sizes(k.n, k.p, k.terms, k.leaf, k.dtype, &tb, &wb) != FMMFFT_OK) return 0;
-    const size_t esize = k.dtype == FMMFFT_C64 ? 8 : 16;
+    const size_t esize = k.dtype == FMMFFT_C128 ? 16 : 8;
     *ws_bytes = wb * (size_t)maxb;
     size_t per_slot = *ws_bytes;
     if (gathered) per_slot += 2 * (size_t)maxb * (size_t)k.n * esize;
@@ -182,7 +182,7 @@ Shape *fmmfft_server_s::get_shape(const Key &k, int &err) {
     sh->key = k;
     sh->maxb = maxb;
     sh->gathered = gathered;
-    sh->esize = k.dtype == FMMFFT_C64 ? 8 : 16;
+    sh->esize = k.dtype == FMMFFT_C128 ? 16 : 8;
     int rc = fmmfft_plan_create(&sh->plan, k.n, k.p, k.terms, k.leaf, k.dtype, 1);
     bool ok = rc == FMMFFT_OK;
     for (auto &sl : sh->slots) {

@@ -40,7 +40,10 @@ typedef struct fmmfft_plan_s *fmmfft_plan;
 enum fmmfft_dtype {
     FMMFFT_C64 = 0,  /* complex64 storage, FP32 contractions, FP64 setup,
                         section sums, combine and inter-pass twiddles    */
-    FMMFFT_C128 = 1  /* complex128 end to end                           */
+    FMMFFT_C128 = 1, /* complex128 end to end                           */
+    FMMFFT_C64_SPLIT = 2 /* FMMFFT_C64 with the M2L flips as split products:
+                            each FP64 operator carried as FP32 hi + lo and
+                            both applied (twice the flip MFMAs)           */
 };
 
 enum fmmfft_status {

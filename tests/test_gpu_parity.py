@@ -283,14 +283,15 @@ def tones_input(n, seed):
 
 def test_fp32_precision_form_full_size(fmm, orc, torch):
     """N=2^26, P=64 (M=2^20), p=16, leaf 64, 16 tones + noise: the complex64
-    precision form to rel-L2 <= 1e-6 and its complex128 comparator (the
+    precision form (plain and with split-product M2L) to rel-L2 <= 1e-6 and
+    its complex128 comparator (the
     plain form the headline ratio divides by) to <= 1e-11, both against the
     oracle's FP64 FFT of the same (complex64-rounded) input."""
     say("fp32 full size N=2^26")
     n = 1 << 26
     x = tones_input(n, 3).astype(np.complex64).astype(np.complex128)
     Y = orc.fft64(x)
-    for dtype, tol in ((fmm.C64, 1e-6), (fmm.C128, 1e-11)):
+    for dtype, tol in ((fmm.C64, 1e-6), (fmm.C64_SPLIT, 1e-6), (fmm.C128, 1e-11)):
         plan, X = run_plan(fmm, torch, x, n, 64, 16, 64, dtype)
         plan.close()
         err = rel(X, Y)
@@ -367,3 +368,55 @@ def test_reference_recorded_max_error(fmm, torch, N, P, B, T, pct):
     # there, the GPU's value sits 1.5 % lower.  So the unrounded value must
     # print the recorded string itself or after a relative change of <= 2 %.
     assert any("%f" % (val * f) == pct for f in (1.0, 0.98, 1.02)), (val, pct)
+
+
+# ---------------------------------------------------------------------------
+# FMMFFT_C64_SPLIT: the M2L flips as FP32 hi + lo split products
+# ---------------------------------------------------------------------------
+
+@pytest.mark.parametrize("n,p,terms,leaf", [(1 << 16, 64, 16, 64), (1 << 18, 32, 16, 64), (1 << 15, 8, 20, 32)])
+def test_split_m2l_form(fmm, orc, torch, n, p, terms, leaf):
+    """Same multipole tree as the plain complex64 plan (bit for bit: the
+    upward pass is unchanged), local expansions that differ from the plain
+    plan's (the residual products are applied) and stay within 1e-5 of the
+    oracle's FP64 tree at every level, the top flip on the GPU's own FP32
+    level-2 expansions as close to the same flip in FP64 as the plain plan's
+    (the operator's FP32 rounding, ~2^-24, is removed; the FP32 accumulation
+    and the FP32 inputs dominate, so the two agree to within noise), and
+    the transform within 1e-6 of the FP64 FFT."""
+    say(f"split m2l n={n} p={p} terms={terms} leaf={leaf}")
+    x = orc.normal(n, 9).astype(np.complex64).astype(np.complex128)
+    ref = orc.OraclePlan(n, p, terms, leaf)
+    S = p - 1
+    phi_ref = ref.upward(x)
+    psi_ref = ref.downward(phi_ref)
+    tl = ref.tree_len()
+    depth = (n // p // leaf).bit_length() - 1
+    trees, outs = {}, {}
+    for dt in (fmm.C64, fmm.C64_SPLIT):
+        plan, X = run_plan(fmm, torch, x, n, p, terms, leaf, dt)
+        outs[dt] = X
+        for which in (1, 2):
+            raw = plan.read_stage(which, 2 * tl, np.float32).astype(np.float64).reshape(-1, terms, S, 2)
+            trees[dt, which] = (raw[..., 0] + 1j * raw[..., 1]).transpose(0, 2, 1).reshape(-1)
+        plan.close()
+    assert np.array_equal(trees[fmm.C64, 1], trees[fmm.C64_SPLIT, 1])
+    errs = {}
+    for dt in (fmm.C64, fmm.C64_SPLIT):
+        got = trees[dt, 2]
+        for lv in range(2, depth + 1):
+            lo, cnt = ref.level_off(lv), (1 << lv) * terms * S
+            errs[dt, lv] = rel(got[lo:lo + cnt], psi_ref[lo:lo + cnt])
+            assert errs[dt, lv] < 1e-5, (dt, lv)
+    # top flip in FP64 on the GPU's level-2 multipole expansions (shared by
+    # both plans): psi_2[q] = topflp . phi_2[(q + 2) mod 4]
+    lo2, cnt2 = ref.level_off(2), 4 * terms * S
+    phi2 = trees[fmm.C64, 1][lo2:lo2 + cnt2].reshape(4, S, terms)
+    want = np.stack([phi2[(q + 2) % 4] @ ref.topflp.T for q in range(4)]).reshape(-1)
+    e_plain = rel(trees[fmm.C64, 2][lo2:lo2 + cnt2], want)
+    e_split = rel(trees[fmm.C64_SPLIT, 2][lo2:lo2 + cnt2], want)
+    say("  top flip vs FP64 on the same inputs: plain %.3e split %.3e" % (e_plain, e_split))
+    assert e_split < 1.1 * e_plain and e_split < 2e-7
+    assert not np.array_equal(trees[fmm.C64, 2], trees[fmm.C64_SPLIT, 2])
+    Y = orc.fft64(x)
+    assert rel(outs[fmm.C64_SPLIT], Y) < 1e-6
