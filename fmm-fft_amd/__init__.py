@@ -99,7 +99,8 @@ def lib():
         L.fmmfft_profile_begin.argtypes = [vp, i32]
         L.fmmfft_profile_end.argtypes = [vp, ctypes.POINTER(ctypes.c_double), i32, ctypes.POINTER(i64)]
         L.fmmfft_server_create.argtypes = [ctypes.POINTER(vp), sz, i32]
-        L.fmmfft_server_submit.argtypes = [vp, i64, i32, i32, i32, i32, i32, vp, vp, ctypes.POINTER(ctypes.c_uint64)]
+        L.fmmfft_server_submit.argtypes = [vp, i64, i32, i32, i32, i32, i32, vp, vp, vp,
+                                           ctypes.POINTER(ctypes.c_uint64)]
         L.fmmfft_server_wait.argtypes = [vp, ctypes.c_uint64, ctypes.POINTER(ctypes.c_double)]
         L.fmmfft_server_drain.argtypes = [vp]
         L.fmmfft_server_stats_get.argtypes = [vp, ctypes.POINTER(ServerStats)]
@@ -275,11 +276,13 @@ class Server:
         self._h = h
 
     def submit(self, x, out, n: int, p_blocks: int, terms: int, leaf: int, dtype: int,
-               direction: int = FORWARD) -> int:
+               direction: int = FORWARD, stream=None) -> int:
+        """Queue FFT(x) -> out; x is read after the work queued so far on
+        ``stream`` (torch's current stream by default)."""
         t = ctypes.c_uint64()
         _check(lib().fmmfft_server_submit(self._h, n, p_blocks, terms, leaf, dtype, direction,
                                           ctypes.c_void_p(x.data_ptr()), ctypes.c_void_p(out.data_ptr()),
-                                          ctypes.byref(t)), "fmmfft_server_submit")
+                                          _stream_ptr(stream), ctypes.byref(t)), "fmmfft_server_submit")
         return t.value
 
     def wait(self, ticket: int) -> float:

@@ -11,6 +11,7 @@
 #include <new>
 
 #include "common.hpp"
+#include "internal.hpp"
 #include "kernels_fft.hpp"
 #include "kernels_fmm.hpp"
 #include "kernels_ops.hpp"
@@ -829,6 +830,15 @@ int fmmfft_plan_sizes(int64_t n, int32_t p_blocks, int32_t terms, int32_t leaf, 
 
 int fmmfft_plan_create(fmmfft_plan *out, int64_t n, int32_t p_blocks, int32_t terms, int32_t leaf,
                        int32_t dtype, int32_t max_batch) {
+    return fmmfft::plan_create_impl(out, n, p_blocks, terms, leaf, dtype, max_batch, true);
+}
+
+}  // extern "C"
+
+// Plan creation; with_workspace = false skips the default workspace (for
+// callers that always pass their own, fmmfft_execute_ws).
+int fmmfft::plan_create_impl(fmmfft_plan *out, int64_t n, int32_t p_blocks, int32_t terms, int32_t leaf,
+                             int32_t dtype, int32_t max_batch, bool with_workspace) {
     if (!out) return FMMFFT_EINVAL;
     *out = nullptr;
     if (max_batch < 0) return FMMFFT_EINVAL;
@@ -932,11 +942,13 @@ int fmmfft_plan_create(fmmfft_plan *out, int64_t n, int32_t p_blocks, int32_t te
     (void)hipStreamDestroy(st);
     PLAN_TRY(e1);
     PLAN_TRY(e2);
-    PLAN_TRY(hipMalloc(&p->work, p->ws.stride * (size_t)p->max_batch));
+    if (with_workspace) PLAN_TRY(hipMalloc(&p->work, p->ws.stride * (size_t)p->max_batch));
 #undef PLAN_TRY
     *out = p;
     return FMMFFT_OK;
 }
+
+extern "C" {
 
 static void prof_free(fmmfft_plan plan) {
     if (plan->prof_ev) {
@@ -1024,7 +1036,7 @@ int fmmfft_execute_ws(fmmfft_plan plan, int64_t count, const void *d_in, void *d
 
 int fmmfft_execute_batched(fmmfft_plan plan, int64_t count, const void *d_in, void *d_out, int32_t dir,
                            void *stream) {
-    if (!plan || count <= 0 || count > plan->max_batch) return FMMFFT_EINVAL;
+    if (!plan || !plan->work || count <= 0 || count > plan->max_batch) return FMMFFT_EINVAL;
     return fmmfft_execute_ws(plan, count, d_in, d_out, dir, plan->work, plan->ws.stride * (size_t)count,
                              stream);
 }
@@ -1078,7 +1090,7 @@ int fmmfft_read_table(fmmfft_plan plan, int32_t which, void *host_dst, size_t by
 }
 
 int fmmfft_read_stage(fmmfft_plan plan, int32_t which, void *host_dst, size_t bytes) {
-    if (!plan || !host_dst) return FMMFFT_EINVAL;
+    if (!plan || !host_dst || !plan->work) return FMMFFT_EINVAL;
     const Geom &g = plan->prm.g;
     const WsLayout &wl = plan->ws;
     size_t off, cnt;

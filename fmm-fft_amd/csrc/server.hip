@@ -4,24 +4,37 @@
 // fmmfft_server_*).  The reference has no counterpart (it runs one transform
 // per MPI job, src/fmmfft.c:1192-1288).
 //
-// One scheduler thread owns every HIP call of the server.  Requests carry
-// caller-owned device buffers.  Shapes with N <= kSlotElems are gathered
-// into a contiguous slot (one device-to-device copy per request), executed
-// as one batched launch sequence captured once per (slot, count, direction)
-// as a hipGraph, and scattered back; larger shapes execute directly on the
-// caller's buffers.  Each shape owns two slots (two workspaces), so one
-// batch can be filled and copied while the previous one computes; different
-// shapes run on different streams and overlap on the device.
+// One scheduler thread owns every shape, slot, plan and graph, and makes
+// every HIP call of the server except the submit-time event record.  The
+// server mutex guards only what submitters and waiters share with it (the
+// pending queue, the tickets, the statistics): the scheduler picks a batch
+// under the lock and makes its HIP calls (plan creation, allocation, graph
+// capture, launches) with the lock released.
+//
+// A request carries caller-owned device buffers and the caller's stream: an
+// event recorded on that stream at submit time is waited for on the slot
+// stream before the request's input is read.  Shapes with N <= kSlotElems
+// are gathered into a contiguous slot (one device-to-device copy per
+// request), executed as one batched launch sequence captured once per
+// (slot, count) as a hipGraph, and scattered back; larger shapes execute
+// directly on the caller's buffers.  Each shape owns two slots (two
+// workspaces), so one batch is filled and copied while the previous one
+// computes, and batches of different shapes run on different streams and
+// overlap on the device.  The scheduler launches the oldest pending request
+// whose shape has a free slot (a busy shape does not hold back the others).
+// Completion times come from a timing event recorded after each batch,
+// measured against a reference event whose host time is known, so they do
+// not depend on when the scheduler notices them.
 #include "../../include/fmmfft.h"
 
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <condition_variable>
 #include <deque>
-#include <list>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -30,6 +43,8 @@
 #include <tuple>
 #include <unordered_map>
 #include <vector>
+
+#include "internal.hpp"
 
 namespace {
 
@@ -48,6 +63,7 @@ struct Key {
     bool operator<(const Key &o) const {
         return std::tie(n, p, terms, leaf, dtype, dir) < std::tie(o.n, o.p, o.terms, o.leaf, o.dtype, o.dir);
     }
+    bool operator==(const Key &o) const { return !(*this < o) && !(o < *this); }
 };
 
 struct Req {
@@ -55,6 +71,7 @@ struct Req {
     Key key;
     const void *in;
     void *out;
+    hipEvent_t ready = nullptr;  // recorded on the caller's stream at submit
     double t_submit = 0, t_done = 0;
     int status = FMMFFT_OK;
     bool done = false;
@@ -62,7 +79,7 @@ struct Req {
 
 struct Slot {
     hipStream_t st = nullptr;
-    hipEvent_t ev = nullptr;
+    hipEvent_t ev = nullptr;  // timing event after the batch's last operation
     void *ws = nullptr, *buf_in = nullptr, *buf_out = nullptr;
     size_t ws_bytes = 0;
     bool busy = false;
@@ -78,31 +95,42 @@ struct Shape {
     size_t esize = 8, bytes = 0;
     Slot slots[kSlots];
     double last_used = 0;
+    Slot *free_slot() {
+        for (auto &s : slots)
+            if (!s.busy) return &s;
+        return nullptr;
+    }
 };
 
 }  // namespace
 
 struct fmmfft_server_s {
-    size_t budget, used = 0, peak = 0;
+    size_t budget;
+    std::atomic<size_t> used{0}, peak{0};
     int max_batch;
+    int device = 0;
+    // shared with submitters and waiters (guarded by mu)
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
     std::deque<std::shared_ptr<Req>> pending;
     std::unordered_map<uint64_t, std::shared_ptr<Req>> reqs;
-    std::map<Key, std::unique_ptr<Shape>> shapes;
+    std::vector<hipEvent_t> event_pool;
     uint64_t next_id = 1;
     bool stop = false;
-    std::thread worker;
-    int device = 0;
-    // statistics
+    int64_t inflight = 0;
     std::vector<double> lat_ms;
     double first_submit = 0, last_done = 0, gflop = 0;
-    int64_t batches = 0, plans_created = 0, plans_evicted = 0, graphs_captured = 0, inflight = 0;
+    int64_t batches = 0, plans_created = 0, plans_evicted = 0, graphs_captured = 0;
+    // owned by the scheduler thread
+    std::map<Key, std::unique_ptr<Shape>> shapes;
+    hipEvent_t ref_ev = nullptr;  // completed reference event ...
+    double ref_t = 0;             // ... and the host time it completed at
+    std::thread worker;
 
     void run();
-    bool try_launch(std::unique_lock<std::mutex> &lk);
-    void retire(Shape &sh, Slot &sl);
+    bool launch_one();
     bool poll_completions();
+    void finish(const std::vector<std::shared_ptr<Req>> &batch, int status);
     Shape *get_shape(const Key &k, int &err);
     void free_shape(Shape &sh);
     bool evict_one(const Shape *keep);
@@ -110,6 +138,8 @@ struct fmmfft_server_s {
 
 namespace {
 
+// device memory the server holds for a shape: tables, and per slot the
+// workspace for maxb signals plus (gathered shapes) the in/out buffers
 size_t shape_bytes(const Key &k, int maxb, bool gathered, size_t *ws_bytes) {
     size_t tb = 0, wb = 0;
     if (fmmfft_plan_sizes(k.n, k.p, k.terms, k.leaf, k.dtype, &tb, &wb) != FMMFFT_OK) return 0;
@@ -150,13 +180,14 @@ bool fmmfft_server_s::evict_one(const Shape *keep) {
     if (!victim) return false;
     free_shape(*victim);
     shapes.erase(victim->key);
+    std::lock_guard<std::mutex> g(mu);
     ++plans_evicted;
     return true;
 }
 
-// Plan + workspaces (+ gather buffers) for a shape, created under the budget.
-// err: FMMFFT_OK, FMMFFT_ENOMEM (does not fit even alone: reject) or -1
-// (does not fit now: wait for completions).
+// Plan (tables only) + workspaces (+ gather buffers) for a shape, created
+// under the budget.  err: FMMFFT_OK, FMMFFT_ENOMEM / EINVAL / ENOTSUP (cannot
+// run even alone: reject) or -1 (does not fit now: wait for completions).
 Shape *fmmfft_server_s::get_shape(const Key &k, int &err) {
     err = FMMFFT_OK;
     auto it = shapes.find(k);
@@ -183,14 +214,14 @@ Shape *fmmfft_server_s::get_shape(const Key &k, int &err) {
     sh->maxb = maxb;
     sh->gathered = gathered;
     sh->esize = k.dtype == FMMFFT_C128 ? 16 : 8;
-    int rc = fmmfft_plan_create(&sh->plan, k.n, k.p, k.terms, k.leaf, k.dtype, 1);
+    // tables only: every execute passes a slot workspace
+    int rc = fmmfft::plan_create_impl(&sh->plan, k.n, k.p, k.terms, k.leaf, k.dtype, maxb, false);
     bool ok = rc == FMMFFT_OK;
     for (auto &sl : sh->slots) {
         if (!ok) break;
         sl.ws_bytes = wsb;
         ok = hipStreamCreateWithFlags(&sl.st, hipStreamNonBlocking) == hipSuccess &&
-             hipEventCreateWithFlags(&sl.ev, hipEventDisableTiming) == hipSuccess &&
-             hipMalloc(&sl.ws, wsb) == hipSuccess;
+             hipEventCreate(&sl.ev) == hipSuccess && hipMalloc(&sl.ws, wsb) == hipSuccess;
         if (ok && gathered) {
             const size_t bb = (size_t)maxb * (size_t)k.n * sh->esize;
             ok = hipMalloc(&sl.buf_in, bb) == hipSuccess && hipMalloc(&sl.buf_out, bb) == hipSuccess;
@@ -203,85 +234,139 @@ Shape *fmmfft_server_s::get_shape(const Key &k, int &err) {
         err = rc == FMMFFT_EINVAL || rc == FMMFFT_ENOTSUP ? rc : FMMFFT_ENOMEM;
         return nullptr;
     }
-    peak = std::max(peak, used);
-    ++plans_created;
+    size_t pk = peak.load();
+    while (used > pk && !peak.compare_exchange_weak(pk, used.load())) {
+    }
+    {
+        std::lock_guard<std::mutex> g(mu);
+        ++plans_created;
+    }
     Shape *raw = sh.get();
     shapes[k] = std::move(sh);
     return raw;
 }
 
-void fmmfft_server_s::retire(Shape &sh, Slot &sl) {
+// Mark a batch finished (status of every request) and wake the waiters.
+// Failed requests complete at the current time; completed batches carry
+// their event-measured completion time already.
+void fmmfft_server_s::finish(const std::vector<std::shared_ptr<Req>> &batch, int status) {
     const double t = now_s();
-    for (auto &r : sl.batch) {
-        r->t_done = t;
+    std::lock_guard<std::mutex> g(mu);
+    inflight -= (int64_t)batch.size();
+    for (auto &r : batch) {
+        if (r->ready) event_pool.push_back(r->ready);
+        r->ready = nullptr;
+        r->status = status;
+        if (status != FMMFFT_OK || r->t_done == 0) r->t_done = t;
         r->done = true;
-        lat_ms.push_back((r->t_done - r->t_submit) * 1e3);
-        gflop += 5.0 * (double)r->key.n * std::log2((double)r->key.n) * 1e-9;
+        if (status == FMMFFT_OK) {
+            lat_ms.push_back(std::max(0.0, r->t_done - r->t_submit) * 1e3);
+            gflop += 5.0 * (double)r->key.n * std::log2((double)r->key.n) * 1e-9;
+            last_done = std::max(last_done, r->t_done);
+        }
     }
-    last_done = t;
-    inflight -= (int64_t)sl.batch.size();
-    sl.batch.clear();
-    sl.busy = false;
-    sh.last_used = t;
+    cv_done.notify_all();
 }
 
-// Retire every finished batch; returns true if any finished.
+// Retire every finished batch (completion time from its timing event);
+// returns true if any finished.
 bool fmmfft_server_s::poll_completions() {
     bool any = false;
     for (auto &kv : shapes)
-        for (auto &sl : kv.second->slots)
-            if (sl.busy && hipEventQuery(sl.ev) == hipSuccess) {
-                retire(*kv.second, sl);
-                any = true;
-            }
-    if (any) cv_done.notify_all();
+        for (auto &sl : kv.second->slots) {
+            if (!sl.busy || hipEventQuery(sl.ev) != hipSuccess) continue;
+            float ms = 0.f;
+            double t = now_s();
+            if (hipEventElapsedTime(&ms, ref_ev, sl.ev) == hipSuccess) t = ref_t + 1e-3 * (double)ms;
+            for (auto &r : sl.batch) r->t_done = t;
+            auto batch = std::move(sl.batch);
+            sl.batch.clear();
+            sl.busy = false;
+            kv.second->last_used = now_s();
+            finish(batch, FMMFFT_OK);
+            any = true;
+        }
     return any;
 }
 
-// Launch one batch: the oldest pending request's shape, up to maxb requests
-// of that shape in arrival order.  Returns false if nothing could start.
-bool fmmfft_server_s::try_launch(std::unique_lock<std::mutex> &lk) {
-    (void)lk;
-    if (pending.empty()) return false;
-    const Key key = pending.front()->key;
+// Launch one batch: the oldest pending request whose shape has a free slot
+// (or has no plan yet), with up to maxb requests of that shape in arrival
+// order.  Returns false if nothing could start.
+bool fmmfft_server_s::launch_one() {
+    std::vector<std::shared_ptr<Req>> batch;
+    Key key{};
+    bool have = false;
+    {
+        std::lock_guard<std::mutex> g(mu);
+        std::vector<Key> busy;  // shapes seen without a free slot
+        for (auto it = pending.begin(); it != pending.end() && !have; ++it) {
+            const Key &k = (*it)->key;
+            if (std::find(busy.begin(), busy.end(), k) != busy.end()) continue;
+            auto sit = shapes.find(k);
+            if (sit == shapes.end() || sit->second->free_slot()) {
+                key = k;
+                have = true;
+            } else {
+                busy.push_back(k);
+            }
+        }
+        if (!have) return false;
+        auto sit = shapes.find(key);
+        const int maxb = sit == shapes.end() ? 1 : sit->second->maxb;
+        // (a shape without a plan yet: its first request now, the rest
+        // once the plan exists).  Taken requests count as in flight until
+        // they finish or go back to the queue.
+        for (auto it = pending.begin(); it != pending.end() && (int)batch.size() < maxb;) {
+            if ((*it)->key == key) {
+                batch.push_back(*it);
+                it = pending.erase(it);
+            } else {
+                ++it;
+            }
+        }
+        inflight += (int64_t)batch.size();
+    }
     int err = FMMFFT_OK;
     Shape *sh = get_shape(key, err);
     if (!sh) {
-        if (err == -1) return false;  // wait for memory
-        auto r = pending.front();
-        pending.pop_front();
-        r->status = err;
-        r->done = true;
-        cv_done.notify_all();
+        if (err == -1) {
+            // no room now: back to the front of the queue, in order
+            std::lock_guard<std::mutex> g(mu);
+            for (auto it = batch.rbegin(); it != batch.rend(); ++it) pending.push_front(*it);
+            inflight -= (int64_t)batch.size();
+            return false;
+        }
+        finish(batch, err);
         return true;
     }
-    Slot *sl = nullptr;
-    for (auto &s : sh->slots)
-        if (!s.busy) {
-            sl = &s;
-            break;
+    if ((int)batch.size() < sh->maxb) {
+        // a new shape: fill the batch up to its real size
+        std::lock_guard<std::mutex> g(mu);
+        const size_t before = batch.size();
+        for (auto it = pending.begin(); it != pending.end() && (int)batch.size() < sh->maxb;) {
+            if ((*it)->key == key) {
+                batch.push_back(*it);
+                it = pending.erase(it);
+            } else {
+                ++it;
+            }
         }
-    if (!sl) return false;
-    // gather the batch
-    std::vector<std::shared_ptr<Req>> batch;
-    for (auto it = pending.begin(); it != pending.end() && (int)batch.size() < sh->maxb;) {
-        if (!((*it)->key < key) && !(key < (*it)->key)) {
-            batch.push_back(*it);
-            it = pending.erase(it);
-        } else {
-            ++it;
-        }
+        inflight += (int64_t)(batch.size() - before);
     }
+    Slot *sl = sh->free_slot();
     const int cnt = (int)batch.size();
     const size_t sig_bytes = (size_t)key.n * sh->esize;
     int rc = FMMFFT_OK;
-    if (sh->gathered) {
-        for (int i = 0; i < cnt; ++i)
+    // inputs are complete on the callers' streams
+    for (auto &r : batch)
+        if (r->ready && hipStreamWaitEvent(sl->st, r->ready, 0) != hipSuccess) rc = FMMFFT_EHIP;
+    if (rc == FMMFFT_OK && sh->gathered) {
+        for (int i = 0; i < cnt && rc == FMMFFT_OK; ++i)
             if (hipMemcpyAsync((char *)sl->buf_in + i * sig_bytes, batch[i]->in, sig_bytes, hipMemcpyDeviceToDevice,
                                sl->st) != hipSuccess)
                 rc = FMMFFT_EHIP;
         auto g = sl->graphs.find(cnt);
-        if (g == sl->graphs.end()) {
+        if (rc == FMMFFT_OK && g == sl->graphs.end()) {
             hipGraph_t graph = nullptr;
             hipGraphExec_t exec = nullptr;
             if (hipStreamBeginCapture(sl->st, hipStreamCaptureModeThreadLocal) == hipSuccess) {
@@ -291,6 +376,7 @@ bool fmmfft_server_s::try_launch(std::unique_lock<std::mutex> &lk) {
                 if (erc == FMMFFT_OK && ce == hipSuccess &&
                     hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) == hipSuccess) {
                     sl->graphs[cnt] = exec;
+                    std::lock_guard<std::mutex> lk(mu);
                     ++graphs_captured;
                 } else {
                     rc = FMMFFT_EHIP;
@@ -307,48 +393,43 @@ bool fmmfft_server_s::try_launch(std::unique_lock<std::mutex> &lk) {
             if (hipMemcpyAsync(batch[i]->out, (char *)sl->buf_out + i * sig_bytes, sig_bytes,
                                hipMemcpyDeviceToDevice, sl->st) != hipSuccess)
                 rc = FMMFFT_EHIP;
-    } else {
+    } else if (rc == FMMFFT_OK) {
         rc = fmmfft_execute_ws(sh->plan, 1, batch[0]->in, batch[0]->out, key.dir, sl->ws, sl->ws_bytes, sl->st);
     }
     if (rc == FMMFFT_OK && hipEventRecord(sl->ev, sl->st) != hipSuccess) rc = FMMFFT_EHIP;
     if (rc != FMMFFT_OK) {
         (void)hipStreamSynchronize(sl->st);
-        for (auto &r : batch) {
-            r->status = rc;
-            r->done = true;
-        }
-        cv_done.notify_all();
+        finish(batch, rc);
         return true;
     }
     sl->batch = std::move(batch);
     sl->busy = true;
     sh->last_used = now_s();
-    inflight += cnt;
+    std::lock_guard<std::mutex> g(mu);
     ++batches;
     return true;
 }
 
 void fmmfft_server_s::run() {
     (void)hipSetDevice(device);
-    std::unique_lock<std::mutex> lk(mu);
+    // reference event for completion times: its host time is taken right
+    // after it completes
+    if (hipEventCreate(&ref_ev) == hipSuccess && hipEventRecord(ref_ev, nullptr) == hipSuccess &&
+        hipEventSynchronize(ref_ev) == hipSuccess)
+        ref_t = now_s();
     while (true) {
-        poll_completions();
+        const bool done_any = poll_completions();
         bool launched = false;
-        while (try_launch(lk)) launched = true;
+        while (launch_one()) launched = true;
+        std::unique_lock<std::mutex> lk(mu);
         if (stop && pending.empty() && inflight == 0) break;
-        if (launched) continue;
-        if (inflight > 0) {
-            // nothing can start: wait for the oldest in-flight batch
-            hipEvent_t ev = nullptr;
-            for (auto &kv : shapes)
-                for (auto &sl : kv.second->slots)
-                    if (sl.busy && !ev) ev = sl.ev;
-            lk.unlock();
-            if (ev) (void)hipEventSynchronize(ev);
-            lk.lock();
-        } else {
+        if (launched || done_any) continue;
+        if (inflight > 0)
+            // batches in flight: look again shortly, or as soon as a request
+            // arrives
+            cv_work.wait_for(lk, std::chrono::microseconds(20));
+        else
             cv_work.wait(lk, [&] { return stop || !pending.empty(); });
-        }
     }
 }
 
@@ -371,7 +452,8 @@ int fmmfft_server_create(fmmfft_server *out, size_t hbm_budget_bytes, int32_t ma
 }
 
 int fmmfft_server_submit(fmmfft_server srv, int64_t n, int32_t p_blocks, int32_t terms, int32_t leaf,
-                         int32_t dtype, int32_t dir, const void *d_in, void *d_out, uint64_t *ticket) {
+                         int32_t dtype, int32_t dir, const void *d_in, void *d_out, void *stream,
+                         uint64_t *ticket) {
     if (!srv || !d_in || !d_out || !ticket) return FMMFFT_EINVAL;
     if (dir != FMMFFT_FORWARD && dir != FMMFFT_INVERSE) return FMMFFT_EINVAL;
     int rc = fmmfft_plan_sizes(n, p_blocks, terms, leaf, dtype, nullptr, nullptr);
@@ -383,6 +465,19 @@ int fmmfft_server_submit(fmmfft_server srv, int64_t n, int32_t p_blocks, int32_t
     {
         std::lock_guard<std::mutex> g(srv->mu);
         if (srv->stop) return FMMFFT_EINVAL;
+        if (!srv->event_pool.empty()) {
+            r->ready = srv->event_pool.back();
+            srv->event_pool.pop_back();
+        }
+    }
+    if (!r->ready && hipEventCreateWithFlags(&r->ready, hipEventDisableTiming) != hipSuccess) return FMMFFT_EHIP;
+    if (hipEventRecord(r->ready, static_cast<hipStream_t>(stream)) != hipSuccess) {
+        std::lock_guard<std::mutex> g(srv->mu);
+        srv->event_pool.push_back(r->ready);
+        return FMMFFT_EHIP;
+    }
+    {
+        std::lock_guard<std::mutex> g(srv->mu);
         r->id = srv->next_id++;
         r->t_submit = now_s();
         if (srv->first_submit == 0) srv->first_submit = r->t_submit;
@@ -402,7 +497,7 @@ int fmmfft_server_wait(fmmfft_server srv, uint64_t ticket, double *latency_ms) {
     auto r = it->second;
     srv->cv_done.wait(lk, [&] { return r->done; });
     srv->reqs.erase(ticket);
-    if (latency_ms) *latency_ms = (r->t_done - r->t_submit) * 1e3;
+    if (latency_ms) *latency_ms = std::max(0.0, r->t_done - r->t_submit) * 1e3;
     return r->status;
 }
 
@@ -460,6 +555,8 @@ int fmmfft_server_destroy(fmmfft_server srv) {
     if (srv->worker.joinable()) srv->worker.join();
     for (auto &kv : srv->shapes) srv->free_shape(*kv.second);
     srv->shapes.clear();
+    for (hipEvent_t e : srv->event_pool) (void)hipEventDestroy(e);
+    if (srv->ref_ev) (void)hipEventDestroy(srv->ref_ev);
     delete srv;
     return FMMFFT_OK;
 }

@@ -157,13 +157,19 @@ int fmmfft_profile_end(fmmfft_plan plan, double *stage_ms, int32_t nstages, int6
  * per MPI job, src/fmmfft.c:1192-1288).  A scheduler thread admits requests
  * of mixed (N, P, terms, leaf, dtype, dir) as they arrive, keeps plans and
  * workspaces cached on the device under an HBM budget (evicting idle shapes,
- * least recently used first), batches same-shape requests into one launch
- * sequence replayed from a captured hipGraph (N <= 2^20; the requests are
- * gathered into and scattered from a contiguous slot), runs larger shapes
- * directly on the caller's buffers, and overlaps batches of different shapes
- * on separate streams.  Requests carry caller-owned device buffers that must
- * stay valid until fmmfft_server_wait returns; every result equals what
- * fmmfft_execute returns for the same input.
+ * least recently used first; a request that does not fit waits, at the head
+ * of the queue, until completions free memory), batches same-shape requests
+ * into one launch sequence replayed from a captured hipGraph (N <= 2^20; the
+ * requests are gathered into and scattered from a contiguous slot), runs
+ * larger shapes directly on the caller's buffers, and overlaps batches of
+ * different shapes on separate streams: it launches the oldest request whose
+ * shape has a free slot.  Requests carry caller-owned device buffers that
+ * must stay valid until fmmfft_server_wait returns; every result equals what
+ * fmmfft_execute returns for the same input.  bytes_in_use / peak_bytes
+ * count every device allocation the server makes for its shapes (plan
+ * tables, slot workspaces, gather buffers), all charged to the budget.
+ * Latency is submit to completion on the device (a timing event after each
+ * batch), not to when the scheduler notices it.
  * ------------------------------------------------------------------------- */
 typedef struct fmmfft_server_s *fmmfft_server;
 
@@ -176,11 +182,14 @@ typedef struct fmmfft_server_stats {
 } fmmfft_server_stats;
 
 int fmmfft_server_create(fmmfft_server *out, size_t hbm_budget_bytes, int32_t max_batch);
-/* Queue one transform; returns at once with a ticket.  EINVAL / ENOTSUP for
- * parameters fmmfft_plan_create would reject; a shape that cannot fit the
- * budget even alone completes with FMMFFT_ENOMEM. */
+/* Queue one transform; returns at once with a ticket.  stream: the caller's
+ * hipStream_t that produces d_in (NULL = default stream): an event recorded
+ * on it now is waited for on the device before d_in is read.  EINVAL /
+ * ENOTSUP for parameters fmmfft_plan_create would reject; a shape that
+ * cannot fit the budget even alone completes with FMMFFT_ENOMEM. */
 int fmmfft_server_submit(fmmfft_server srv, int64_t n, int32_t p_blocks, int32_t terms, int32_t leaf,
-                         int32_t dtype, int32_t dir, const void *d_in, void *d_out, uint64_t *ticket);
+                         int32_t dtype, int32_t dir, const void *d_in, void *d_out, void *stream,
+                         uint64_t *ticket);
 /* Block until the ticket's transform has finished on the device; returns its
  * status and latency.  A ticket can be waited for once. */
 int fmmfft_server_wait(fmmfft_server srv, uint64_t ticket, double *latency_ms);

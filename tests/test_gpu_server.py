@@ -3,6 +3,8 @@ every result of a mixed request stream equals the direct execute of the same
 input (bit for bit: batched and single launches run the same arithmetic),
 requests that cannot fit the HBM budget fail cleanly, idle shapes are
 evicted to admit new ones, and the statistics add up."""
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -124,6 +126,11 @@ def test_request_larger_than_budget_fails_cleanly(fmm, torch):
     with pytest.raises(fmm.FmmFftError) as e:
         srv.wait(tk)
     assert e.value.status == fmm.ENOMEM
+    # the failed request's latency is reported (not negative)
+    tk = srv.submit(x, out, n, p, t, leaf, dt)
+    lat = ctypes.c_double(-1.0)
+    assert fmm.lib().fmmfft_server_wait(srv._h, tk, ctypes.byref(lat)) == fmm.ENOMEM
+    assert 0.0 <= lat.value < 60e3
     with pytest.raises(fmm.FmmFftError):
         srv.submit(x, out, n, 3, t, leaf, dt)  # invalid P
     srv.close()
@@ -141,17 +148,49 @@ def test_idle_shapes_are_evicted(fmm, torch):
     na = ta + 2 * wa + 4 * a[0] * 8
     nb = tb + 2 * wb + 4 * b[0] * 16
     budget = max(na, nb) + min(na, nb) // 2
-    srv = fmm.Server(budget, max_batch=1)
     outs = []
     for i in range(6):
         shape = a if i % 2 == 0 else b
         dtype = torch.complex64 if shape[4] == 0 else torch.complex128
         x = torch.randn(shape[0], dtype=dtype, device="cuda")
-        o = torch.empty_like(x)
+        outs.append((shape, x, torch.empty_like(x)))
+    torch.cuda.synchronize()
+    free0, _ = torch.cuda.mem_get_info()
+    srv = fmm.Server(budget, max_batch=1)
+    low = free0
+    for shape, x, o in outs:
         srv.wait(srv.submit(x, o, *shape))
-        outs.append((shape, x, o))
+        low = min(low, torch.cuda.mem_get_info()[0])
     st = srv.stats()
     srv.close()
     assert st["plans_evicted"] >= 1 and st["peak_bytes"] <= budget
+    # what the device really lost while the server ran stays within the
+    # budget (plus a little for the runtime's own stream, event and graph
+    # objects): every allocation of the server is charged
+    say(f"  device memory used {(free0 - low) / 2**20:.1f} MiB, budget {budget / 2**20:.1f} MiB")
+    assert free0 - low <= budget + (32 << 20)
     for shape, x, o in outs:
         assert torch.equal(o, fmm.Plan(*shape).execute(x))
+
+
+def test_submit_waits_for_the_callers_stream(fmm, torch):
+    """The input is produced on a side stream behind a long queue of work and
+    submitted without synchronizing: the server must wait for that stream."""
+    say("server stream order")
+    n, p, t, leaf, dt = shape_for(18, 12, False)
+    side = torch.cuda.Stream()
+    a = torch.randn(4096, 4096, device="cuda")
+    src = torch.randn(n, dtype=torch.complex64, device="cuda")
+    x = torch.zeros_like(src)
+    out = torch.empty_like(src)
+    torch.cuda.synchronize()
+    srv = fmm.Server(4 << 30, max_batch=4)
+    with torch.cuda.stream(side):
+        for _ in range(20):
+            a = a @ a * 1e-3  # keeps the side stream busy for a while
+        x.copy_(src)
+        tk = srv.submit(x, out, n, p, t, leaf, dt, stream=side)
+    srv.wait(tk)
+    srv.close()
+    torch.cuda.synchronize()
+    assert torch.equal(out, fmm.Plan(n, p, t, leaf, dt).execute(src))
