@@ -87,7 +87,9 @@ int validate(int64_t n, int32_t P, int32_t terms, int32_t leaf, int32_t dtype, P
     if (!is_pow2(s) || s < 8) return FMMFFT_EINVAL;
     if (leaf % 4) return FMMFFT_ENOTSUP;                   // leaf lanes own 4 targets
     if ((int64_t)leaf * P > (is_c64(dtype) ? 8192 : 4096)) return FMMFFT_ENOTSUP;
-    if (!is_pow2(M) || M > (int64_t(1) << 20)) return FMMFFT_ENOTSUP;  // two FFT passes of <= 1024
+    // local FFTs in at most three passes of <= 1024 points; offsets inside
+    // one signal fit 32 bits
+    if (!is_pow2(M) || M > (int64_t(1) << 30) || n > (int64_t(1) << 30)) return FMMFFT_ENOTSUP;
     Geom &g = out->g;
     g.N = n;
     g.M = M;
@@ -107,28 +109,41 @@ int validate(int64_t n, int32_t P, int32_t terms, int32_t leaf, int32_t dtype, P
 struct TableLayout {
     size_t cheb, initch, ops, topflp, evalm, evalmh, kv, evfrag, opfrag, n_ops;  // contiguous block
     size_t gfac, wP;                                             // complex128 [P]
-    size_t twA, twB, tlo, thi;                                   // complex128
+    size_t twA, twB, twC, tlo, thi;                              // complex128
     size_t total;                                                // doubles
 };
 
+// Local FFT of length M in 1, 2 or 3 passes of sub-FFTs of length <= 1024:
+// M = M1 (one pass), M1 M2 (M <= 2^20) or M1 M2 M3 (M <= 2^30; the first
+// pass transforms along the M1 axis with the rest, R = M2 M3, as columns).
 struct FftLayout {
-    bool two_pass;
-    int64_t M1, M2;  // two-pass split (M1 * M2 = M); single pass: M1 = M
+    bool two_pass;    // more than one pass
+    int passes;
+    int64_t M1, M2, M3;
     int logR;
 };
 
 FftLayout fft_layout(const Geom &g) {
     FftLayout f{};
     const int m = ilog2(g.M);
+    f.M2 = f.M3 = 1;
     if (g.M <= 1024) {
-        f.two_pass = false;
+        f.passes = 1;
         f.M1 = g.M;
-        f.M2 = 1;
-    } else {
-        f.two_pass = true;
+    } else if (m <= 20) {
+        f.passes = 2;
         f.M1 = int64_t(1) << ((m + 1) / 2);
         f.M2 = int64_t(1) << (m / 2);
+    } else {
+        // m = 21..30: three factors of 2^7..2^10 (the last >= the column
+        // tile width of the middle pass)
+        f.passes = 3;
+        const int m1 = (m + 2) / 3, m2 = (m - m1 + 1) / 2, m3 = m - m1 - m2;
+        f.M1 = int64_t(1) << m1;
+        f.M2 = int64_t(1) << m2;
+        f.M3 = int64_t(1) << m3;
     }
+    f.two_pass = f.passes > 1;
     f.logR = (m + 1) / 2;
     return f;
 }
@@ -153,6 +168,7 @@ TableLayout table_layout(const Geom &g, const FftLayout &f) {
     t.wP = o; o += 2 * (size_t)g.P;
     t.twA = o; o += 2 * (size_t)f.M1;  // full circle: e^{-2 pi i k / M1}, k < M1
     t.twB = o; o += 2 * (size_t)f.M2;
+    t.twC = o; o += 2 * (size_t)f.M3;
     const int64_t R = int64_t(1) << f.logR;
     t.tlo = o; o += 2 * (size_t)R;
     t.thi = o; o += 2 * (size_t)std::max<int64_t>(1, g.M / R);
@@ -262,8 +278,9 @@ Shapes shapes(const Geom &g, size_t esize) {
         if (fl.two_pass) {
             const int cols = esize == 4 ? fft_cols<float>((int)fl.M1) : fft_cols<double>((int)fl.M1);
             const int tw = std::max(cols, (int)(128 / (2 * esize)));
-            // (tw <= 32 <= M2 whenever two passes run; kept as a guard)
-            sh.rm = tw <= fl.M2 ? RowMap{ilog2(tw), ilog2(fl.M1), ilog2(fl.M2)} : RowMap{-1, 0, 0};
+            // (tw <= 32 <= M / M1 whenever several passes run; kept as a guard)
+            const int64_t R = g.M / fl.M1;
+            sh.rm = tw <= R ? RowMap{ilog2(tw), ilog2(fl.M1), ilog2(R)} : RowMap{-1, 0, 0};
         } else {
             sh.rm = RowMap{-1, 0, 0};
         }
@@ -315,7 +332,7 @@ struct fmmfft_plan_s {
     double *tab64 = nullptr;  // FP64 tables (TableLayout offsets)
     float *tab32 = nullptr;   // FP32 copies of the operator block + twiddles
     _Float16 *kfrag = nullptr;  // FP16 split near-field fragments (leaf_h16)
-    size_t tw32_A = 0, tw32_B = 0;  // float offsets of FP32 stage twiddles
+    size_t tw32_A = 0, tw32_B = 0, tw32_C = 0;  // float offsets of FP32 stage twiddles
     size_t opf_lo32 = 0;            // float offset of the FP32 residual operator fragments
     void *work = nullptr;     // default workspace
     int64_t max_batch = 1;
@@ -329,7 +346,7 @@ namespace {
 
 template <typename T>
 struct TabPtrs {
-    const T *initch, *ops, *topflp, *evalm, *evalmh, *kv, *evfrag, *opfrag, *twA, *twB;
+    const T *initch, *ops, *topflp, *evalm, *evalmh, *kv, *evfrag, *opfrag, *twA, *twB, *twC;
     const T *opfrag_lo;  // FP32 residual of the fragment-ordered operators (complex64), else null
 };
 
@@ -340,13 +357,14 @@ template <>
 TabPtrs<double> tab_ptrs<double>(const fmmfft_plan_s *p) {
     const double *t = p->tab64;
     return {t + p->tl.initch, t + p->tl.ops, t + p->tl.topflp, t + p->tl.evalm, t + p->tl.evalmh,
-            t + p->tl.kv,     t + p->tl.evfrag, t + p->tl.opfrag, t + p->tl.twA, t + p->tl.twB, nullptr};
+            t + p->tl.kv,     t + p->tl.evfrag, t + p->tl.opfrag, t + p->tl.twA, t + p->tl.twB, t + p->tl.twC,
+            nullptr};
 }
 template <>
 TabPtrs<float> tab_ptrs<float>(const fmmfft_plan_s *p) {
     const float *t = p->tab32;
     return {t + p->tl.initch, t + p->tl.ops, t + p->tl.topflp, t + p->tl.evalm, t + p->tl.evalmh,
-            t + p->tl.kv,     t + p->tl.evfrag, t + p->tl.opfrag, t + p->tw32_A, t + p->tw32_B,
+            t + p->tl.kv,     t + p->tl.evfrag, t + p->tl.opfrag, t + p->tw32_A, t + p->tw32_B, t + p->tw32_C,
             p->prm.dtype == FMMFFT_C64_SPLIT ? t + p->opf_lo32 : nullptr};
 }
 
@@ -765,8 +783,10 @@ int run(const fmmfft_plan_s *p, int64_t count, const void *d_in, void *d_out, in
         launch_fft<T>(f.M1, g, fp, nz, st, out, out, g.N, g.N, twA, tlo, thi);
         mark(6);
     } else {
+        // first pass: sub-FFTs of length M1 along the columns of [M1][R]
+        const int64_t R = g.M / f.M1;
         FftPass a{};
-        a.F = f.M2;
+        a.F = R;
         if (sh.rm.ltw >= 0) {
             // column-tiled input (the leaf's row map, tiles of TW >= C
             // columns): column f0 + j of tile q = f0 / TW at
@@ -774,20 +794,46 @@ int run(const fmmfft_plan_s *p, int64_t count, const void *d_in, void *d_out, in
             a.in_fs = 1; a.in_es = int64_t(1) << sh.rm.ltw;
             a.in_bs = f.M1 << sh.rm.ltw; a.in_ltw = sh.rm.ltw;
         } else {
-            a.in_fs = 1; a.in_es = f.M2; a.in_bs = 0; a.in_ltw = 30;
+            a.in_fs = 1; a.in_es = R; a.in_bs = 0; a.in_ltw = 30;
         }
-        a.out_fs = 1; a.out_ks = f.M2;
+        a.out_fs = 1; a.out_ks = R;
         a.twiddle = 1;
         a.logR = f.logR;
         launch_fft<T>(f.M1, g, a, nz, st, out, psec, g.N, wstride_cx, twA, tlo, thi);
         mark(6);
-        FftPass bq{};
-        bq.F = f.M1;
-        bq.in_fs = f.M2; bq.in_es = 1; bq.in_bs = 0; bq.in_ltw = 30;
-        bq.out_fs = 1; bq.out_ks = f.M1;
-        bq.conj_out = conj;
-        bq.logR = f.logR;
-        launch_fft<T>(f.M2, g, bq, nz, st, psec, out, wstride_cx, g.N, twB, tlo, thi);
+        if (f.passes == 2) {
+            FftPass bq{};
+            bq.F = f.M1;
+            bq.in_fs = f.M2; bq.in_es = 1; bq.in_bs = 0; bq.in_ltw = 30;
+            bq.out_fs = 1; bq.out_ks = f.M1;
+            bq.conj_out = conj;
+            bq.logR = f.logR;
+            launch_fft<T>(f.M2, g, bq, nz, st, psec, out, wstride_cx, g.N, twB, tlo, thi);
+        } else {
+            // Z[k1][r], r = n2 M3 + n3.  Second pass, in place: for every
+            // (k1, n3) (sub-FFT f = k1 M3 + n3, a column of row k1) the
+            // M2-point FFT over n2, times w_R^{n3 k2} = w_M^{M1 n3 k2}
+            const int l1 = ilog2(f.M1), l3 = ilog2(f.M3);
+            const cx<T> *twC = reinterpret_cast<const cx<T> *>(tb.twC);
+            FftPass b2{};
+            b2.F = f.M1 * f.M3;
+            b2.in_fs = 1; b2.in_es = f.M3; b2.in_bs = R; b2.in_ltw = l3;
+            b2.out_fs = 1; b2.out_ks = f.M3; b2.out_bs = R; b2.out_tile = l3;
+            b2.twiddle = 1; b2.tw_lf = l3; b2.tw_shift = l1;
+            b2.logR = f.logR;
+            launch_fft<T>(f.M2, g, b2, nz, st, psec, psec, wstride_cx, wstride_cx, twB, tlo, thi);
+            // third pass: for every (k1, k2) (sub-FFT f = k2 M1 + k1: k1
+            // fastest, so that the outputs X[k1 + M1 k2 + M1 M2 k3] of
+            // neighbouring sub-FFTs are neighbours) the M3-point FFT of the
+            // contiguous row at k1 R + k2 M3
+            FftPass b3{};
+            b3.F = f.M1 * f.M2;
+            b3.in_fs = f.M2 * f.M3; b3.in_es = 1; b3.in_bs = f.M3; b3.in_ltw = l1;
+            b3.out_fs = 1; b3.out_ks = f.M1 * f.M2;
+            b3.conj_out = conj;
+            b3.logR = f.logR;
+            launch_fft<T>(f.M3, g, b3, nz, st, psec, out, wstride_cx, g.N, twC, tlo, thi);
+        }
     }
     mark(7);
     hipError_t e = hipGetLastError();
@@ -818,7 +864,7 @@ int fmmfft_plan_sizes(int64_t n, int32_t p_blocks, int32_t terms, int32_t leaf, 
     if (table_bytes) {
         size_t t = tl.total * sizeof(double);
         const size_t tp = 4 * (size_t)leaf_kt(prm.g.terms), nopf = ((size_t)(prm.g.n - 2) * NOPS + 1) * tp * tp;
-        if (is_c64(dtype)) t += (tl.n_ops + 2 * f.M1 + 2 * f.M2) * sizeof(float);
+        if (is_c64(dtype)) t += (tl.n_ops + 2 * f.M1 + 2 * f.M2 + 2 * f.M3) * sizeof(float);
         if (dtype == FMMFFT_C64_SPLIT) t += nopf * sizeof(float);
         if (is_c64(dtype) && sh.leaf_h16)
             t += (size_t)prm.g.S * (3 * prm.g.b / 16 + prm.g.b / 16 - 1) * 2 * 64 * 4 * sizeof(_Float16);
@@ -893,7 +939,7 @@ int fmmfft::plan_create_impl(fmmfft_plan *out, int64_t n, int32_t p_blocks, int3
                            (const double *)(t + tl.topflp), t + tl.opfrag, tp);
     }
     const int64_t R = int64_t(1) << p->fft.logR;
-    const int64_t MA = p->fft.M1, MB = p->fft.M2;
+    const int64_t MA = p->fft.M1, MB = p->fft.M2, MC = p->fft.M3;
     hipLaunchKernelGGL(k_build_twiddles, dim3(blocks_for(MA + R + g.M / R)), dim3(256), 0, st, MA, g.M, R,
                        reinterpret_cast<double2 *>(t + tl.twA), reinterpret_cast<double2 *>(t + tl.tlo),
                        reinterpret_cast<double2 *>(t + tl.thi));
@@ -901,14 +947,19 @@ int fmmfft::plan_create_impl(fmmfft_plan *out, int64_t n, int32_t p_blocks, int3
         hipLaunchKernelGGL(k_build_twiddles, dim3(blocks_for(MB)), dim3(256), 0, st, MB, g.M, int64_t(0),
                            reinterpret_cast<double2 *>(t + tl.twB), reinterpret_cast<double2 *>(t + tl.tlo),
                            reinterpret_cast<double2 *>(t + tl.thi));
+    if (p->fft.passes == 3)
+        hipLaunchKernelGGL(k_build_twiddles, dim3(blocks_for(MC)), dim3(256), 0, st, MC, g.M, int64_t(0),
+                           reinterpret_cast<double2 *>(t + tl.twC), reinterpret_cast<double2 *>(t + tl.tlo),
+                           reinterpret_cast<double2 *>(t + tl.thi));
     if (is_c64(prm.dtype)) {
-        const size_t nA = 2 * (size_t)MA, nB = 2 * (size_t)MB;
+        const size_t nA = 2 * (size_t)MA, nB = 2 * (size_t)MB, nC = 2 * (size_t)MC;
         const size_t tp = 4 * (size_t)leaf_kt(g.terms);
         const size_t nopf = prm.dtype == FMMFFT_C64_SPLIT ? ((size_t)(g.n - 2) * NOPS + 1) * tp * tp : 0;
         p->tw32_A = tl.n_ops;
         p->tw32_B = tl.n_ops + nA;
-        p->opf_lo32 = tl.n_ops + nA + nB;
-        const size_t n32 = tl.n_ops + nA + nB + nopf;
+        p->tw32_C = tl.n_ops + nA + nB;
+        p->opf_lo32 = tl.n_ops + nA + nB + nC;
+        const size_t n32 = tl.n_ops + nA + nB + nC + nopf;
         hipError_t e = hipMalloc(&p->tab32, n32 * sizeof(float));
         if (e != hipSuccess) {
             (void)hipStreamDestroy(st);
@@ -921,6 +972,9 @@ int fmmfft::plan_create_impl(fmmfft_plan *out, int64_t n, int32_t p_blocks, int3
         if (p->fft.two_pass)
             hipLaunchKernelGGL(k_to_f32, dim3(blocks_for(nB)), dim3(256), 0, st, t + tl.twB,
                                p->tab32 + p->tw32_B, (int64_t)nB);
+        if (p->fft.passes == 3)
+            hipLaunchKernelGGL(k_to_f32, dim3(blocks_for(nC)), dim3(256), 0, st, t + tl.twC,
+                               p->tab32 + p->tw32_C, (int64_t)nC);
         if (nopf)
             hipLaunchKernelGGL(k_to_f32_lo, dim3(blocks_for(nopf)), dim3(256), 0, st, t + tl.opfrag,
                                p->tab32 + p->opf_lo32, (int64_t)nopf);

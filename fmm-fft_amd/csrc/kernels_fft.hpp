@@ -24,7 +24,10 @@ struct FftPass {
                            // k2*M + q*in_bs + (r + j)*in_fs + e*in_es
                            // (in_ltw = 30: q = 0, the plain form)
     int64_t out_fs, out_ks;
-    int twiddle;           // 1: multiply output k of sub-FFT f by w_M^{f k}
+    int64_t out_bs;        // column-tiled output (out_tile > 0): sub-FFT f at
+    int out_tile;          // (f >> out_tile)*out_bs + (f & (2^out_tile - 1))*out_fs
+    int twiddle;           // 1: multiply output k of sub-FFT f by w_M^{g(f) k},
+    int tw_lf, tw_shift;   // g(f) = (f mod 2^tw_lf) * 2^tw_shift (tw_lf = 0: f)
     int conj_out;          // conjugate the output (inverse transform)
     int logR;              // split of the pass-1 twiddle exponent
 };
@@ -209,7 +212,7 @@ __device__ inline int64_t xcd_remap(int64_t bid, int64_t nwg) {
 // current one, so HBM latency hides behind the LDS work.  Each lane's output
 // elements share one sub-FFT f and step k by NT/C, so the pass-1 twiddle
 // w_M^{f k} is a per-lane FP64 recurrence started from two table lookups.
-// Offsets inside one signal fit 32 bits (N <= 2^26), which keeps the
+// Offsets inside one signal fit 32 bits (N <= 2^30), which keeps the
 // per-element address arithmetic to one register.
 constexpr int kFftThreads = 512;
 
@@ -290,9 +293,11 @@ __global__ __launch_bounds__(kFftThreads) void k_fft(Geom g, FftPass fp, const c
         constexpr bool TWF = FORM == FFT_PASS1 || sizeof(T) == 8;
         c64 th0{1.0, 0.0}, tl0{1.0, 0.0}, ths{1.0, 0.0}, tls{1.0, 0.0};
         if constexpr (TWF) {
-            const int fa = f0 + (int)threadIdx.x % C;
-            const int e0 = (int)(((int64_t)fa * (threadIdx.x / C)) & (M - 1));
-            const int es = (int)(((int64_t)fa * (NT / C)) & (M - 1));
+            int fa = f0 + (int)threadIdx.x % C;
+            if (fp.tw_lf) fa &= (1 << fp.tw_lf) - 1;
+            const int64_t fs = (int64_t)fa << fp.tw_shift;
+            const int e0 = (int)((fs * (threadIdx.x / C)) & (M - 1));
+            const int es = (int)((fs * (NT / C)) & (M - 1));
             th0 = thi[e0 >> fp.logR];
             tl0 = tlo[e0 & Rm];
             ths = thi[es >> fp.logR];
@@ -327,7 +332,10 @@ __global__ __launch_bounds__(kFftThreads) void k_fft(Geom g, FftPass fp, const c
                 w = cmul(w, ws);
             }
             if (fp.conj_out) val.im = -val.im;
-            if (f < cv) out_s[base + (f0 + f) * fstep_out + k * out_ks] = val;
+            const int fg = f0 + f;
+            const int fo = fp.out_tile ? (fg >> fp.out_tile) * (int)fp.out_bs + (fg & ((1 << fp.out_tile) - 1)) * fstep_out
+                                       : fg * fstep_out;
+            if (f < cv) out_s[base + fo + k * out_ks] = val;
         }
         tt = tn;
     }

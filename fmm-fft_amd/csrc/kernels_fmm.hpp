@@ -646,8 +646,8 @@ void k_leaf_mma(Geom g, const cx<T> *__restrict__ x, int64_t x_stride,
     const int P = g.P, terms = g.terms;
     const int64_t M = g.M;
     const int rowsS = tree_rows(g);      // tree row stride
-    // offsets inside one signal fit 32 bits (N <= 2^26); P, M and s are
-    // powers of two
+    // offsets into x and psec fit 32 bits (N <= 2^30; tree offsets are
+    // 64-bit); P, M and s are powers of two
     const int lp = __builtin_ctz(P), mmask = (int)M - 1, smask = g.s - 1;
     // window rows of box q start at q b G + boxpad(q): for 8 consecutive
     // boxes the pads cover 8 distinct bank pairs apart from the row stride's
@@ -734,7 +734,7 @@ void k_leaf_mma(Geom g, const cx<T> *__restrict__ x, int64_t x_stride,
             const int box = (i0 - 1 + bx9) & smask;
             // padding terms meet zero L2T rows; section 0 reads its own rows
             // and never uses them
-            lr[u] = ps[(box * terms + min(kk, terms - 1)) * rowsS + 2 * sce + (rr & 1)];
+            lr[u] = ps[((int64_t)box * terms + min(kk, terms - 1)) * rowsS + 2 * sce + (rr & 1)];
         }
     };
     auto commit = [&]() __attribute__((always_inline)) {
@@ -1051,7 +1051,7 @@ void k_leaf_h16(Geom g, const c32 *__restrict__ x, int64_t x_stride, const float
             const int kk = rem / (2 * G), rr = rem % (2 * G);
             const int sce = sc0 + (rr >> 1);
             const int box = (i0 - 1 + bx9) & smask;
-            lr[u] = ps[(box * terms + min(kk, terms - 1)) * rowsS + 2 * sce + (rr & 1)];
+            lr[u] = ps[((int64_t)box * terms + min(kk, terms - 1)) * rowsS + 2 * sce + (rr & 1)];
         }
     };
     // window rows to LDS (imaginary parts negated for the inverse) and the

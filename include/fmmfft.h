@@ -67,9 +67,13 @@ enum fmmfft_status {
  * getvlen, src/fmmfft.c:1110-1183).  table_bytes: device operator tables;
  * work_bytes: device workspace for ONE signal.  Either pointer may be NULL.
  * Validity rules (the reference checks none of them, src/fmmfft.c:1200-1212):
- * N % P == 0, P a power of two with 2 <= P <= 64, M = N/P divisible by leaf,
- * s = M/leaf a power of two >= 8, 2 <= terms <= 32, 2 <= leaf <= 128,
- * leaf * P <= 8192 (complex64) / 4096 (complex128). */
+ * FMMFFT_EINVAL unless N % P == 0, P a power of two with 2 <= P <= 64,
+ * M = N/P divisible by leaf, s = M/leaf a power of two >= 8,
+ * 2 <= terms <= 32, 2 <= leaf <= 128; FMMFFT_ENOTSUP (valid, but outside
+ * what the kernels take) unless leaf % 4 == 0, leaf * P <= 8192 (complex64)
+ * / 4096 (complex128), M a power of two and N <= 2^30 (the local FFTs run
+ * in up to three passes of <= 1024 points; offsets into one signal are
+ * 32-bit). */
 int fmmfft_plan_sizes(int64_t n, int32_t p_blocks, int32_t terms, int32_t leaf,
                       int32_t dtype, size_t *table_bytes, size_t *work_bytes);
 
@@ -144,7 +148,8 @@ int fmmfft_read_stage(fmmfft_plan plan, int32_t which, void *host_dst, size_t by
  * for benchmarks (no reference counterpart; the reference times whole mft()
  * calls with omp_get_wtime, src/fmmfft.c:1259-1263).  Stages:
  * 0 S2M + section sums, 1 upward pass, 2 downward pass, 3 leaf evaluation
- * (L2T + near field), 4 combine, 5 local FFT pass 1, 6 local FFT pass 2.
+ * (L2T + near field), 4 combine, 5 local FFT pass 1, 6 the remaining local FFT passes (none
+ * for M <= 1024, one for M <= 2^20, two above).
  * profile_begin creates events for up to max_calls executes; profile_end
  * synchronises, writes the summed milliseconds of each stage over the
  * recorded executes and their count, and stops recording. */

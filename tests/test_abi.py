@@ -60,13 +60,22 @@ def test_workspace_scales_with_precision(fmm):
         (1 << 16, 4, 16, 64, 7, 1),      # bad dtype
         (1 << 16, 4, 16, 6, 1, 1),       # M % leaf != 0
         (1 << 14, 4, 16, 2, 1, 4),       # leaf not a multiple of 4
-        (1 << 26, 2, 16, 64, 1, 4),      # M > 2^24
+        (1 << 31, 2, 16, 64, 1, 4),      # N > 2^30
+        (3 << 20, 2, 16, 64, 1, 1),      # M = 3 * 2^19: s not a power of two
     ],
 )
 def test_plan_sizes_rejects_invalid(fmm, n, p, terms, leaf, dtype, status):
     with pytest.raises(fmm.FmmFftError) as e:
         fmm.plan_sizes(n, p, terms, leaf, dtype)
     assert e.value.status == status
+
+
+def test_plan_sizes_accepts_the_recorded_large_shapes(fmm):
+    """The reference's recorded runs go to M = 2^26 (N = 2^27, P = 2) and
+    N = 2^29 (P = 4), b = 32, T = 15 (three-pass local FFTs here)."""
+    for n, p in [(1 << 27, 2), (1 << 29, 4), (1 << 28, 2), (1 << 30, 64)]:
+        tb, wb = fmm.plan_sizes(n, p, 15, 32, fmm.C128)
+        assert tb > 0 and wb >= n * 16
 
 
 def test_strerror(fmm):

@@ -144,6 +144,8 @@ SMALL = [
     (1 << 18, 64, 16, 32),
     (1 << 16, 16, 16, 16),  # leaf 16: S2M tiles of many boxes
     (1 << 15, 32, 20, 16),
+    (1 << 22, 2, 16, 32),   # M = 2^21: three-pass local FFT (2^7 x 2^7 x 2^7)
+    (1 << 23, 4, 16, 64),   # M = 2^21 with P = 4
 ]
 
 
@@ -166,6 +168,27 @@ def test_fp32_transform(fmm, orc, torch, n, p, terms, leaf):
     Y = orc.fft64(x)
     err = rel(X, Y)
     say(f"  rel-L2 {err:.3e}")
+    assert err < 1e-6
+
+
+@pytest.mark.parametrize("n,p,terms,leaf", [(1 << 24, 8, 16, 64), (1 << 23, 2, 20, 32)])
+def test_three_pass_local_fft(fmm, orc, torch, n, p, terms, leaf):
+    """M = 2^21 (2^7 x 2^7 x 2^7) and 2^22 (2^8 x 2^7 x 2^7): complex128
+    against the oracle's FMM-FFT (1e-12) and FP64 FFT (1e-11), complex64
+    within 1e-6 of the FP64 FFT, and the inverse direction."""
+    say(f"three-pass n={n} p={p} terms={terms} leaf={leaf}")
+    x = orc.normal(n, 31).astype(np.complex64).astype(np.complex128)
+    Y = orc.fft64(x)
+    plan, X = run_plan(fmm, torch, x, n, p, terms, leaf, fmm.C128)
+    assert rel(X, orc.OraclePlan(n, p, terms, leaf).fmmfft(x)) < 1e-12
+    assert rel(X, Y) < 1e-11
+    xd = to_dev(torch, x, plan.torch_dtype)
+    Xi = plan.execute(xd, direction=fmm.INVERSE).cpu().numpy()
+    assert rel(Xi, orc.fft64(x, +1)) < 1e-11
+    plan.close()
+    _, X32 = run_plan(fmm, torch, x, n, p, terms, leaf, fmm.C64)
+    err = rel(X32, Y)
+    say(f"  complex64 rel-L2 {err:.3e}")
     assert err < 1e-6
 
 
@@ -334,7 +357,7 @@ def _reference_rows():
     with open(path) as f:
         rows = list(csv.DictReader(f))
     return [(int(r["N"]), int(r["P"]), int(r["B"]), int(r["T"]), r["max_rel_err_pct"])
-            for r in rows if int(r["N"]) // int(r["P"]) <= (1 << 20)]
+            for r in rows]
 
 
 def ramp_dft(n, m):
@@ -356,18 +379,23 @@ def test_reference_recorded_max_error(fmm, torch, N, P, B, T, pct):
     plan = fmm.Plan(N, P, T, B, fmm.C128)
     x = torch.arange(1, N + 1, dtype=torch.float64, device="cuda").to(torch.complex128)
     M = N // P
-    X = plan.execute(x)[:M].cpu().numpy()
+    X = plan.execute(x, x)[:M].cpu().numpy()  # in place: N = 2^29 is 8 GiB per buffer
+    plan.close()
+    del x
     Y = ramp_dft(N, M)
     val = float(np.max(np.abs(X - Y) / np.abs(Y)) * 100)
     say(f"  max error {val:.9f} % (recorded {pct})")
-    # The recorded column is rounded to 1e-6 %.  From N = 2^23 on, the
-    # ramp's values (up to 2^25) make FP64 rounding visible in this metric:
-    # at (2^23, P=8) and (2^24, P=32) the GPU's and the oracle's outputs
-    # differ by rel-L2 1.7e-15, which at the max-error bin (k = M-1) is
-    # 1.5 % of the error itself; the oracle prints the recorded strings
-    # there, the GPU's value sits 1.5 % lower.  So the unrounded value must
-    # print the recorded string itself or after a relative change of <= 2 %.
-    assert any("%f" % (val * f) == pct for f in (1.0, 0.98, 1.02)), (val, pct)
+    # The recorded column is rounded to 1e-6 %.  From N = 2^23 on (ramp
+    # values up to 2^29) the GPU's value sits 1.2-1.8 % below the recorded
+    # one at every row: at (2^23, P=8) the GPU's and the oracle's outputs
+    # differ by rel-L2 1.7e-15, but that difference is concentrated in the
+    # bins next to the block edges where the error peaks and it lowers the
+    # error there (rms error against the exact DFT 3.250e-3 GPU, 3.289e-3
+    # oracle); the oracle prints the recorded strings.  So the unrounded
+    # value must print the recorded string itself or after a relative change
+    # of at most 2.5 %.
+    lo, hi = float(pct) - 5e-7, float(pct) + 5e-7  # values that print as pct
+    assert "%f" % val == pct or (val * 1.025 >= lo and val * 0.975 < hi), (val, pct)
 
 
 # ---------------------------------------------------------------------------

@@ -28,6 +28,8 @@ CASES = [
     (1 << 15, 4, 20, 128),
     (1 << 17, 16, 15, 128),
     (1 << 18, 32, 12, 128),  # leaf * P = 4096
+    (1 << 22, 2, 9, 64),     # three-pass local FFT (M = 2^21)
+    (1 << 23, 2, 14, 16),    # three-pass, leaf 16 (M = 2^22)
 ]
 
 
