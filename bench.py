@@ -210,7 +210,9 @@ def other_configs(torch, fmm):
     """Throughput of the other GPU workloads: the plain FP64 drop-in (one
     N=2^24 complex128 transform, P=32, p=20, leaf 64, N(0,1) input) and the
     batched precision form (256 signals of N=2^20 complex64, P=16, p=12,
-    leaf 32, in one launch sequence), 5 timed steps after 2 warm-ups each."""
+    leaf 32, in one launch sequence), 5 timed steps after 2 warm-ups each;
+    and the reference's largest recorded shape (N=2^30 complex128, P=64,
+    b=32, T=15; three-pass local FFTs), 3 steps after 1 warm-up."""
     res = {}
     g = torch.Generator(device="cuda").manual_seed(5)
     n = 1 << 24
@@ -229,6 +231,18 @@ def other_configs(torch, fmm):
     res["batched_256x2e20_c64"] = {"gflops": round(count * 5 * n * math.log2(n) / (ms * 1e-3) / 1e9, 1),
                                    "ms_per_batch": round(ms, 4)}
     del plan, x, out
+    torch.cuda.empty_cache()
+    # the reference's largest recorded run: N = 2^30, 64 ranks, b = 32,
+    # T = 15, complex128 (17.8 s median on its CPU cluster,
+    # plotting/raw_data/cluster_out_fmmfft_32_15.txt:2488), in place
+    n = 1 << 30
+    plan = fmm.Plan(n, 64, 15, 32, fmm.C128)
+    x = torch.randn(n, dtype=torch.complex128, device="cuda", generator=g)
+    ms, _, _ = time_plan(torch, plan, x, x, 3, 1, False)
+    res["fp64_n2e30_p64_b32_t15"] = {"gflops": round(5 * n * math.log2(n) / (ms * 1e-3) / 1e9, 1),
+                                     "ms_per_transform": round(ms, 3),
+                                     "reference_cluster_64_ranks_ms": 17800.0}
+    del plan, x
     torch.cuda.empty_cache()
     return res
 
@@ -286,10 +300,9 @@ def leaf_bytes():
     return 2 * N * 8 + s * S * TERMS * 8
 
 
-def path_roofline(stage_ms, esize=8, peak_tflops=PEAK_FP32_TFLOPS):
-    """Roofline time of the whole path at this workload: the sum over stages
-    of max(algorithmic flops / matrix peak, algorithmic bytes / HBM peak),
-    and its ratio to the measured stage times.  Algorithmic counts (a
+def stage_counts(esize=8):
+    """{stage: (algorithmic flops, algorithmic HBM bytes[, compute ms at
+    peak])} of one execute at this workload.  Algorithmic counts (a
     complex x real MAC = 4 flops; a tree level l holds 2^l nodes x S
     sections x p complex terms): S2M reads x, writes tree levels n, n-1
     and, for p <= 16, n-2 (the first M2M levels run in the same kernel);
@@ -340,6 +353,14 @@ def path_roofline(stage_ms, esize=8, peak_tflops=PEAK_FP32_TFLOPS):
         "fft1": (5.0 * N * math.log2(m1), 2 * nb),
         "fft2": (5.0 * N * math.log2(m2), 2 * nb),
     }
+    return count
+
+
+def path_roofline(stage_ms, esize=8, peak_tflops=PEAK_FP32_TFLOPS):
+    """Roofline time of the whole path at this workload: the sum over stages
+    of max(algorithmic flops / matrix peak, algorithmic bytes / HBM peak)
+    (counts from stage_counts), and its ratio to the measured stage times."""
+    count = stage_counts(esize)
     phases, t_roof, t_meas = {}, 0.0, 0.0
     for st, cnt in count.items():
         if st not in stage_ms:
