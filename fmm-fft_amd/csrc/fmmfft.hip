@@ -657,6 +657,9 @@ void s2m_tree_pt(const Geom &g, const Shapes &sh, unsigned nz, hipStream_t st, c
     };
     // downward levels: the split-product M2L instances for FMMFFT_C64_SPLIT
     const bool split = tb.opfrag_lo != nullptr;
+    // complex64 16-term trees whose nodes are whole 64-row groups (P >= 32):
+    // the finest downward levels in the sibling form
+    const bool sib4 = sizeof(T) == 4 && TP == 16 && !pack && tree_rows(g) % 64 == 0;
     auto down = [&](int l, int mode) {
         if (mode == 2) {
             if (split) level(k_tree_mma<T, TP, 2, false, true>, k_tree_mma<T, TP, 2, true, true>, l, 2);
@@ -682,11 +685,14 @@ void s2m_tree_pt(const Geom &g, const Shapes &sh, unsigned nz, hipStream_t st, c
     int l = 3;
     if constexpr (sizeof(T) == 4 && TP == 16) {
         // complex64, 16 terms, nodes of more than two row blocks: levels
-        // 3..n-2 in pairs (l, l + 1) (an odd count leaves level 3 on its
-        // own); the two finest levels run one per launch, which measured
-        // faster than as a pair (0.163 vs 0.181 ms at N = 2^26)
+        // 3..last in pairs (l, l + 1) (an odd count leaves level 3 on its
+        // own), the finer ones one per launch: the two finest as a pair
+        // measured slower (0.181 vs 0.163 ms at N = 2^26), and with whole
+        // 64-row groups the sibling form (k_tree_mma SIB4) takes every level
+        // from 11 on (levels 11..14 at N = 2^26: 0.176 ms downward against
+        // 0.186 with pairs up to level 12)
         if (!pack) {
-            const int last = g.n - 2;
+            const int last = sib4 ? std::min(g.n - 2, 10) : g.n - 2;
             if ((last - 2) % 2 == 1) down(l++, 1);
             for (; l + 1 <= last; l += 2) {
                 const dim3 gd((unsigned)(((int64_t(1) << l) * nrb + 3) / 4), 1, nz);
@@ -699,7 +705,24 @@ void s2m_tree_pt(const Geom &g, const Shapes &sh, unsigned nz, hipStream_t st, c
             }
         }
     }
-    for (; l <= g.n; ++l) down(l, 1);
+    for (; l <= g.n; ++l) {
+        if constexpr (sizeof(T) == 4 && TP == 16) {
+            if (sib4) {
+                // one wave per sibling pair and 64-row group
+                const dim3 gd((unsigned)(((int64_t(1) << (l - 1)) * (tree_rows(g) / 64) + 3) / 4), 1, nz);
+                if (split)
+                    hipLaunchKernelGGL((k_tree_mma<T, TP, 1, false, true, true>), gd, dim3(256), 0, st, g, l,
+                                       (const T *)phi, phi, (const T *)psi, psi, wstride_T, tb.opfrag,
+                                       tb.opfrag_lo);
+                else
+                    hipLaunchKernelGGL((k_tree_mma<T, TP, 1, false, false, true>), gd, dim3(256), 0, st, g, l,
+                                       (const T *)phi, phi, (const T *)psi, psi, wstride_T, tb.opfrag,
+                                       tb.opfrag_lo);
+                continue;
+            }
+        }
+        down(l, 1);
+    }
 }
 
 template <typename T, typename Mark>

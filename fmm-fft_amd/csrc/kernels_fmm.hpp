@@ -1255,7 +1255,18 @@ __host__ __device__ constexpr int tree_mma_rb(int mode) {
 // i.e. P <= 16): a wave takes RB / nrb nodes of one parity (c = 2 RB/nrb
 // (w/2) + w%2 + 2k) with all their row blocks, so small nodes do not cost a
 // wave (and a memory round trip) each.
-template <typename T, int TP, int MODE, bool PACK = false, bool SPLIT_M2L = false>
+//
+// SIB4 (downward, complex64, rows a multiple of 64, the large levels): a
+// wave takes a sibling pair (c, c + 1), c even, for 64 rows, each lane four
+// consecutive rows 64 rg + 4 col + e loaded and stored as one 16-byte access
+// (element e of the vector is column col of N tile e).  The pair reads five
+// distinct sources instead of eight,
+//   c:     shftlp S0 + flip2p S1 + flip2n S3 + flip3n S4
+//   c + 1: shftln S0 + flip3p S1 + flip2p S2 + flip2n S4
+// with S0 = psi_{l-1}[c/2] and S1..S4 = phi_l[c-2], [c-1], [c+2], [c+3],
+// and the 16-byte accesses take a quarter of the address work of the
+// one-row-per-lane form (level 14 at N = 2^26: 107 -> 68 us).
+template <typename T, int TP, int MODE, bool PACK = false, bool SPLIT_M2L = false, bool SIB4 = false>
 __global__ __launch_bounds__(256) void k_tree_mma(Geom g, int l, const T *__restrict__ phi_in,
                                                   T *__restrict__ phi_out, const T *__restrict__ psi_in,
                                                   T *__restrict__ psi_out, int64_t tree_stride,
@@ -1327,6 +1338,68 @@ __global__ __launch_bounds__(256) void k_tree_mma(Geom g, int l, const T *__rest
         }
     };
     T *lvl_out = (MODE == 0 ? phi_out : psi_out) + ts + level_off(g, l);
+    if constexpr (SIB4) {
+        static_assert(sizeof(T) == 4 && MODE == 1 && !PACK && TP == 16, "sibling form: complex64, downward");
+        const int ng = rows >> 6;  // 64-row groups of a node
+        const int pr = wid / ng, c0 = 2 * pr;
+        if (c0 >= nn) return;
+        const int rc = 64 * (wid - pr * ng) + 4 * col;
+        const T *o = opfrag + (size_t)(l - 3) * NOPS * FR;
+        // operators of the even node, then of the odd one (source order above)
+        constexpr int opi[8] = {OP_SHFTLP, OP_FLIP2P, OP_FLIP2N, OP_FLIP3N, OP_SHFTLN, OP_FLIP3P, OP_FLIP2P, OP_FLIP2N};
+        constexpr int se[4] = {0, 1, 3, 4}, so[4] = {0, 1, 2, 4};
+        T a2[8][KS], a2lo[SPLIT ? 8 : 1][KS];
+#pragma unroll
+        for (int t = 0; t < 8; ++t)
+#pragma unroll
+            for (int s = 0; s < KS; ++s) {
+                a2[t][s] = o[opi[t] * FR + s * 64 + lane];
+                if constexpr (SPLIT)
+                    if (t % 4 != 0) a2lo[t][s] = (o - opfrag + opfrag_lo)[opi[t] * FR + s * 64 + lane];
+            }
+        const T *ph = phi_in + ts + level_off(g, l);
+        const T *srcs[5] = {psi_in + ts + level_off(g, l - 1) + (int64_t)pr * node_len,
+                            ph + (int64_t)((c0 - 2) & m) * node_len, ph + (int64_t)((c0 - 1) & m) * node_len,
+                            ph + (int64_t)((c0 + 2) & m) * node_len, ph + (int64_t)((c0 + 3) & m) * node_len};
+        float4 bq[5][KS];
+#pragma unroll
+        for (int t = 0; t < 5; ++t)
+#pragma unroll
+            for (int s = 0; s < KS; ++s)
+                bq[t][s] = *reinterpret_cast<const float4 *>(srcs[t] + min(4 * s + kr, terms - 1) * rows + rc);
+        V4 acc[2][4];
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) acc[h][e] = V4{0, 0, 0, 0};
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+#pragma unroll
+            for (int s = 0; s < KS; ++s)
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const T be = velem(bq[se[t]][s], e), bo = velem(bq[so[t]][s], e);
+                    acc[0][e] = Mfma<T>::mma(a2[t][s], be, acc[0][e]);
+                    acc[1][e] = Mfma<T>::mma(a2[4 + t][s], bo, acc[1][e]);
+                    if constexpr (SPLIT)
+                        if (t != 0) {
+                            acc[0][e] = Mfma<T>::mma(a2lo[t][s], be, acc[0][e]);
+                            acc[1][e] = Mfma<T>::mma(a2lo[4 + t][s], bo, acc[1][e]);
+                        }
+                }
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            T *dst = lvl_out + (int64_t)(c0 + h) * node_len + rc;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int k = Mfma<T>::row(kr, r);
+                if (k < terms)
+                    *reinterpret_cast<float4 *>(dst + (int64_t)k * rows) =
+                        make_float4(acc[h][0][r], acc[h][1][r], acc[h][2][r], acc[h][3][r]);
+            }
+        }
+        return;
+    }
     // all loads of the wave at once, unconditional (a load under a lane
     // condition becomes a branch that waits for it): operator fragments,
     // then the RB row blocks' source fragments at clamped indices.  No
