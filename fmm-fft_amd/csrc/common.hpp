@@ -53,6 +53,26 @@ struct Geom {
     int s, n;       // leaf boxes per section (2^n), tree depth
 };
 
+// Signal sig of a batch in a caller's buffer: base + sig * stride
+// elements, or, for a batch of separate buffers (tab set), tab[sig], a
+// device table of their addresses.  The kernels that read or write callers'
+// buffers take a template flag TAB: without it they use the contiguous form
+// only and take no table load (a per-tile table load, behind a branch or
+// not, made their counter waits cover other loads in flight: +4..16 us per
+// kernel at N = 2^26).
+template <typename E>
+struct SigAddr {
+    E *base;
+    int64_t stride;
+    E *const *tab;
+    template <bool TAB>
+    __device__ inline E *at(int64_t sig) const {
+        if constexpr (TAB)
+            if (tab) return tab[sig];
+        return base + sig * stride;
+    }
+};
+
 // Row order of the leaf output (psec) and of the combine output.  Natural
 // (ltw < 0), or column-tiled for a two-pass local FFT of M = M1 x M2
 // (M1 = 2^lm1, M2 = 2^lm2, row mu = n1 M2 + n2): row mu at

@@ -385,62 +385,62 @@ unsigned resident_grid(const void *kern, int threads, size_t lds, int64_t ntiles
 // one pass of sub-FFTs of length L: persistent when the resident grid is
 // smaller than the tile count (register prefetch of the next tile), else one
 // tile per workgroup (no prefetch)
-template <typename T, int L, int FORM>
-void fft_form(const Geom &g, const FftPass &fp, unsigned nz, hipStream_t st, const cx<T> *in, cx<T> *out,
-              int64_t in_stride, int64_t out_stride, const cx<T> *tw, const c64 *tlo, const c64 *thi,
+template <typename T, int L, int FORM, bool TAB>
+void fft_form(const Geom &g, const FftPass &fp, unsigned nz, hipStream_t st, SigAddr<const cx<T>> in,
+              SigAddr<cx<T>> out, const cx<T> *tw, const c64 *tlo, const c64 *thi,
               bool force_single) {
     constexpr int C = fft_cols<T, L>();
     // tiles of all nz signals (a batch is one persistent grid)
     const int64_t ntot = (fp.F * g.P + C - 1) / C * (int64_t)nz;
     const dim3 block(kFftThreads);
     const unsigned res =
-        force_single ? (unsigned)ntot : resident_grid((const void *)k_fft<T, L, FORM, C, true>, kFftThreads, 0, ntot);
+        force_single ? (unsigned)ntot : resident_grid((const void *)k_fft<T, L, FORM, C, true, TAB>, kFftThreads, 0, ntot);
     if ((int64_t)res < ntot)
-        hipLaunchKernelGGL((k_fft<T, L, FORM, C, true>), dim3(res, 1, 1), block, 0, st, g, fp, in, out, in_stride,
-                           out_stride, tw, tlo, thi, (int)nz);
+        hipLaunchKernelGGL((k_fft<T, L, FORM, C, true, TAB>), dim3(res, 1, 1), block, 0, st, g, fp, in, out,
+                           tw, tlo, thi, (int)nz);
     else
-        hipLaunchKernelGGL((k_fft<T, L, FORM, C, false>), dim3((unsigned)ntot, 1, 1), block, 0, st, g, fp, in, out,
-                           in_stride, out_stride, tw, tlo, thi, (int)nz);
+        hipLaunchKernelGGL((k_fft<T, L, FORM, C, false, TAB>), dim3((unsigned)ntot, 1, 1), block, 0, st, g, fp, in, out,
+                           tw, tlo, thi, (int)nz);
 }
 
-template <typename T, int L>
-void fft_len(const Geom &g, const FftPass &fp, unsigned nz, hipStream_t st, const cx<T> *in, cx<T> *out,
-             int64_t in_stride, int64_t out_stride, const cx<T> *tw, const c64 *tlo, const c64 *thi) {
+template <typename T, int L, bool TAB>
+void fft_len(const Geom &g, const FftPass &fp, unsigned nz, hipStream_t st, SigAddr<const cx<T>> in,
+             SigAddr<cx<T>> out, const cx<T> *tw, const c64 *tlo, const c64 *thi) {
     if (fp.in_es == 1 && fp.out_ks == 1)
-        fft_form<T, L, FFT_SINGLE>(g, fp, nz, st, in, out, in_stride, out_stride, tw, tlo, thi, false);
+        fft_form<T, L, FFT_SINGLE, TAB>(g, fp, nz, st, in, out, tw, tlo, thi, false);
     else if (fp.in_es == 1)
         // FP64 pass 2 measures faster as one tile per workgroup (no
         // persistence): its strided 64-byte output pieces gain from many
         // workgroups in flight more than from the register prefetch
-        fft_form<T, L, FFT_PASS2>(g, fp, nz, st, in, out, in_stride, out_stride, tw, tlo, thi, sizeof(T) == 8);
+        fft_form<T, L, FFT_PASS2, TAB>(g, fp, nz, st, in, out, tw, tlo, thi, sizeof(T) == 8);
     else
-        fft_form<T, L, FFT_PASS1>(g, fp, nz, st, in, out, in_stride, out_stride, tw, tlo, thi, false);
+        fft_form<T, L, FFT_PASS1, TAB>(g, fp, nz, st, in, out, tw, tlo, thi, false);
 }
 
-template <typename T>
-void launch_fft(int64_t L, const Geom &g, const FftPass &fp, unsigned nz, hipStream_t st, const cx<T> *in,
-                cx<T> *out, int64_t in_stride, int64_t out_stride, const cx<T> *tw, const c64 *tlo,
+template <typename T, bool TAB>
+void launch_fft(int64_t L, const Geom &g, const FftPass &fp, unsigned nz, hipStream_t st, SigAddr<const cx<T>> in,
+                SigAddr<cx<T>> out, const cx<T> *tw, const c64 *tlo,
                 const c64 *thi) {
     switch (L) {
-        case 16: fft_len<T, 16>(g, fp, nz, st, in, out, in_stride, out_stride, tw, tlo, thi); break;
-        case 32: fft_len<T, 32>(g, fp, nz, st, in, out, in_stride, out_stride, tw, tlo, thi); break;
-        case 64: fft_len<T, 64>(g, fp, nz, st, in, out, in_stride, out_stride, tw, tlo, thi); break;
-        case 128: fft_len<T, 128>(g, fp, nz, st, in, out, in_stride, out_stride, tw, tlo, thi); break;
-        case 256: fft_len<T, 256>(g, fp, nz, st, in, out, in_stride, out_stride, tw, tlo, thi); break;
-        case 512: fft_len<T, 512>(g, fp, nz, st, in, out, in_stride, out_stride, tw, tlo, thi); break;
-        default: fft_len<T, 1024>(g, fp, nz, st, in, out, in_stride, out_stride, tw, tlo, thi); break;
+        case 16: fft_len<T, 16, TAB>(g, fp, nz, st, in, out, tw, tlo, thi); break;
+        case 32: fft_len<T, 32, TAB>(g, fp, nz, st, in, out, tw, tlo, thi); break;
+        case 64: fft_len<T, 64, TAB>(g, fp, nz, st, in, out, tw, tlo, thi); break;
+        case 128: fft_len<T, 128, TAB>(g, fp, nz, st, in, out, tw, tlo, thi); break;
+        case 256: fft_len<T, 256, TAB>(g, fp, nz, st, in, out, tw, tlo, thi); break;
+        case 512: fft_len<T, 512, TAB>(g, fp, nz, st, in, out, tw, tlo, thi); break;
+        default: fft_len<T, 1024, TAB>(g, fp, nz, st, in, out, tw, tlo, thi); break;
     }
 }
 
-template <typename T>
+template <typename T, bool TAB>
 void launch_combine(const Geom &g, unsigned nz, hipStream_t st, const cx<T> *psec, int64_t ps, const c64 *sumsec,
-                    int64_t ss, const c64 *gfac, const c64 *wP, cx<T> *y, int64_t ys) {
+                    int64_t ss, const c64 *gfac, const c64 *wP, SigAddr<cx<T>> y) {
 #define COMBINE(PP)                                                                                          \
-    hipLaunchKernelGGL((k_combine<T, PP>),                                                                   \
-                       dim3(resident_grid((const void *)k_combine<T, PP>, kCombineThreads, 0,                  \
+    hipLaunchKernelGGL((k_combine<T, PP, TAB>),                                                                   \
+                       dim3(resident_grid((const void *)k_combine<T, PP, TAB>, kCombineThreads, 0,                  \
                                           (g.M + combine_rows<PP>() - 1) / combine_rows<PP>()),                \
                             1, nz),                                                                          \
-                       dim3(kCombineThreads), 0, st, g, psec, ps, sumsec, ss, gfac, wP, y, ys)
+                       dim3(kCombineThreads), 0, st, g, psec, ps, sumsec, ss, gfac, wP, y)
     switch (g.P) {
         case 2: COMBINE(2); break;
         case 4: COMBINE(4); break;
@@ -452,10 +452,10 @@ void launch_combine(const Geom &g, unsigned nz, hipStream_t st, const cx<T> *pse
 #undef COMBINE
 }
 
-template <typename T, int NRT, int G, int KT, int CT>
-void leaf_mma_inst(const Geom &g, const Shapes &sh, unsigned nz, hipStream_t st, const cx<T> *x, const T *psi,
+template <typename T, int NRT, int G, int KT, int CT, bool TAB>
+void leaf_mma_inst(const Geom &g, const Shapes &sh, unsigned nz, hipStream_t st, SigAddr<const cx<T>> x, const T *psi,
                    int64_t wstride_T, cx<T> *psec, int64_t wstride_cx, const TabPtrs<T> &tb, int conj) {
-    auto kern = k_leaf_mma<T, NRT, G, KT, CT>;
+    auto kern = k_leaf_mma<T, NRT, G, KT, CT, TAB>;
     // persistent grid: as many workgroups as can be resident at once
     int dev = 0, cus = 0, per_cu = 0;
     (void)hipGetDevice(&dev);
@@ -471,58 +471,58 @@ void leaf_mma_inst(const Geom &g, const Shapes &sh, unsigned nz, hipStream_t st,
     int64_t nwg64 = std::min<int64_t>(ntot, (int64_t)std::max(cus, 8) * per_cu);
     nwg64 = std::max<int64_t>(ngroups, nwg64 / ngroups * ngroups);
     const unsigned nwg = (unsigned)nwg64;
-    hipLaunchKernelGGL(kern, dim3(nwg, 1, 1), dim3(256), sh.leaf_lds, st, g, x, g.N, psi, wstride_T, psec,
+    hipLaunchKernelGGL(kern, dim3(nwg, 1, 1), dim3(256), sh.leaf_lds, st, g, x, psi, wstride_T, psec,
                        wstride_cx, tb.evfrag, tb.evalm, tb.evalmh, tb.kv, conj, sh.rm, (int)nz);
 }
 
-template <typename T, int NRT, int G>
-void leaf_mma_kt(const Geom &g, const Shapes &sh, unsigned nz, hipStream_t st, const cx<T> *x, const T *psi,
+template <typename T, int NRT, int G, bool TAB>
+void leaf_mma_kt(const Geom &g, const Shapes &sh, unsigned nz, hipStream_t st, SigAddr<const cx<T>> x, const T *psi,
                  int64_t wstride_T, cx<T> *psec, int64_t wstride_cx, const TabPtrs<T> &tb, int conj) {
     // column tiles: NRT 1 -> up to 4, NRT 2 -> up to 2 (leaf_ct); s >= 8
     const int ct = leaf_ct(g);
     if (leaf_kt(g.terms) == 4) {
         if (NRT == 1 && ct == 4)
-            leaf_mma_inst<T, NRT, G, 4, (NRT == 1 ? 4 : 1)>(g, sh, nz, st, x, psi, wstride_T, psec, wstride_cx, tb, conj);
+            leaf_mma_inst<T, NRT, G, 4, (NRT == 1 ? 4 : 1), TAB>(g, sh, nz, st, x, psi, wstride_T, psec, wstride_cx, tb, conj);
         else if (NRT <= 2 && ct >= 2)
-            leaf_mma_inst<T, NRT, G, 4, (NRT <= 2 ? 2 : 1)>(g, sh, nz, st, x, psi, wstride_T, psec, wstride_cx, tb, conj);
+            leaf_mma_inst<T, NRT, G, 4, (NRT <= 2 ? 2 : 1), TAB>(g, sh, nz, st, x, psi, wstride_T, psec, wstride_cx, tb, conj);
         else
-            leaf_mma_inst<T, NRT, G, 4, 1>(g, sh, nz, st, x, psi, wstride_T, psec, wstride_cx, tb, conj);
+            leaf_mma_inst<T, NRT, G, 4, 1, TAB>(g, sh, nz, st, x, psi, wstride_T, psec, wstride_cx, tb, conj);
     } else {
         if (NRT == 1 && ct == 4)
-            leaf_mma_inst<T, NRT, G, 8, (NRT == 1 ? 4 : 1)>(g, sh, nz, st, x, psi, wstride_T, psec, wstride_cx, tb, conj);
+            leaf_mma_inst<T, NRT, G, 8, (NRT == 1 ? 4 : 1), TAB>(g, sh, nz, st, x, psi, wstride_T, psec, wstride_cx, tb, conj);
         else if (NRT <= 2 && ct >= 2)
-            leaf_mma_inst<T, NRT, G, 8, (NRT <= 2 ? 2 : 1)>(g, sh, nz, st, x, psi, wstride_T, psec, wstride_cx, tb, conj);
+            leaf_mma_inst<T, NRT, G, 8, (NRT <= 2 ? 2 : 1), TAB>(g, sh, nz, st, x, psi, wstride_T, psec, wstride_cx, tb, conj);
         else
-            leaf_mma_inst<T, NRT, G, 8, 1>(g, sh, nz, st, x, psi, wstride_T, psec, wstride_cx, tb, conj);
+            leaf_mma_inst<T, NRT, G, 8, 1, TAB>(g, sh, nz, st, x, psi, wstride_T, psec, wstride_cx, tb, conj);
     }
 }
 
-template <typename T, int NRT>
-void leaf_mma_g(const Geom &g, const Shapes &sh, unsigned nz, hipStream_t st, const cx<T> *x, const T *psi,
+template <typename T, int NRT, bool TAB>
+void leaf_mma_g(const Geom &g, const Shapes &sh, unsigned nz, hipStream_t st, SigAddr<const cx<T>> x, const T *psi,
                 int64_t wstride_T, cx<T> *psec, int64_t wstride_cx, const TabPtrs<T> &tb, int conj) {
     if (sh.leaf_G >= 4)
-        leaf_mma_kt<T, NRT, 4>(g, sh, nz, st, x, psi, wstride_T, psec, wstride_cx, tb, conj);
+        leaf_mma_kt<T, NRT, 4, TAB>(g, sh, nz, st, x, psi, wstride_T, psec, wstride_cx, tb, conj);
     else
-        leaf_mma_kt<T, NRT, 2>(g, sh, nz, st, x, psi, wstride_T, psec, wstride_cx, tb, conj);
+        leaf_mma_kt<T, NRT, 2, TAB>(g, sh, nz, st, x, psi, wstride_T, psec, wstride_cx, tb, conj);
 }
 
-template <typename T>
-void launch_leaf_mma(const Geom &g, const Shapes &sh, unsigned nz, hipStream_t st, const cx<T> *x, const T *psi,
+template <typename T, bool TAB>
+void launch_leaf_mma(const Geom &g, const Shapes &sh, unsigned nz, hipStream_t st, SigAddr<const cx<T>> x, const T *psi,
                      int64_t wstride_T, cx<T> *psec, int64_t wstride_cx, const TabPtrs<T> &tb, int conj) {
     switch (g.b / 16) {
-        case 1: leaf_mma_g<T, 1>(g, sh, nz, st, x, psi, wstride_T, psec, wstride_cx, tb, conj); break;
-        case 2: leaf_mma_g<T, 2>(g, sh, nz, st, x, psi, wstride_T, psec, wstride_cx, tb, conj); break;
-        case 4: leaf_mma_g<T, 4>(g, sh, nz, st, x, psi, wstride_T, psec, wstride_cx, tb, conj); break;
-        default: leaf_mma_g<T, 8>(g, sh, nz, st, x, psi, wstride_T, psec, wstride_cx, tb, conj); break;
+        case 1: leaf_mma_g<T, 1, TAB>(g, sh, nz, st, x, psi, wstride_T, psec, wstride_cx, tb, conj); break;
+        case 2: leaf_mma_g<T, 2, TAB>(g, sh, nz, st, x, psi, wstride_T, psec, wstride_cx, tb, conj); break;
+        case 4: leaf_mma_g<T, 4, TAB>(g, sh, nz, st, x, psi, wstride_T, psec, wstride_cx, tb, conj); break;
+        default: leaf_mma_g<T, 8, TAB>(g, sh, nz, st, x, psi, wstride_T, psec, wstride_cx, tb, conj); break;
     }
 }
 
-template <int NRT, int G, int KT>
-void leaf_h16_inst(const fmmfft_plan_s *p, unsigned nz, hipStream_t st, const c32 *x, const float *psi,
+template <int NRT, int G, int KT, bool TAB>
+void leaf_h16_inst(const fmmfft_plan_s *p, unsigned nz, hipStream_t st, SigAddr<const c32> x, const float *psi,
                    int64_t wstride_T, c32 *psec, int64_t wstride_cx, const TabPtrs<float> &tb, int conj) {
     const Geom &g = p->prm.g;
     const Shapes &sh = p->sh;
-    auto kern = k_leaf_h16<NRT, G, KT>;
+    auto kern = k_leaf_h16<NRT, G, KT, TAB>;
     int dev = 0, cus = 0, per_cu = 0;
     (void)hipGetDevice(&dev);
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
@@ -537,38 +537,39 @@ void leaf_h16_inst(const fmmfft_plan_s *p, unsigned nz, hipStream_t st, const c3
     const int64_t ntot = ngroups * (g.s / LeafH16<NRT>::BB) * (int64_t)nz;
     int64_t nwg64 = std::min<int64_t>(ntot, (int64_t)std::max(cus, 8) * per_cu);
     nwg64 = std::max<int64_t>(ngroups, nwg64 / ngroups * ngroups);
-    hipLaunchKernelGGL(kern, dim3((unsigned)nwg64, 1, 1), dim3(nth), sh.leaf_lds, st, g, x, g.N, psi, wstride_T,
+    hipLaunchKernelGGL(kern, dim3((unsigned)nwg64, 1, 1), dim3(nth), sh.leaf_lds, st, g, x, psi, wstride_T,
                        psec, wstride_cx, tb.evfrag, tb.evalm, tb.evalmh, tb.kv,
                        reinterpret_cast<const h16x4 *>(p->kfrag), p->tab64 + p->tl.kv, conj, sh.rm, (int)nz);
 }
 
-template <int NRT, int G>
-void leaf_h16_kt(const fmmfft_plan_s *p, unsigned nz, hipStream_t st, const c32 *x, const float *psi,
+template <int NRT, int G, bool TAB>
+void leaf_h16_kt(const fmmfft_plan_s *p, unsigned nz, hipStream_t st, SigAddr<const c32> x, const float *psi,
                  int64_t wstride_T, c32 *psec, int64_t wstride_cx, const TabPtrs<float> &tb, int conj) {
     if (leaf_kt(p->prm.g.terms) == 4)
-        leaf_h16_inst<NRT, G, 4>(p, nz, st, x, psi, wstride_T, psec, wstride_cx, tb, conj);
+        leaf_h16_inst<NRT, G, 4, TAB>(p, nz, st, x, psi, wstride_T, psec, wstride_cx, tb, conj);
     else
-        leaf_h16_inst<NRT, G, 8>(p, nz, st, x, psi, wstride_T, psec, wstride_cx, tb, conj);
+        leaf_h16_inst<NRT, G, 8, TAB>(p, nz, st, x, psi, wstride_T, psec, wstride_cx, tb, conj);
 }
 
-void launch_leaf_h16(const fmmfft_plan_s *p, unsigned nz, hipStream_t st, const c32 *x, const float *psi,
+template <bool TAB>
+void launch_leaf_h16(const fmmfft_plan_s *p, unsigned nz, hipStream_t st, SigAddr<const c32> x, const float *psi,
                      int64_t wstride_T, c32 *psec, int64_t wstride_cx, const TabPtrs<float> &tb, int conj) {
     const int G = p->sh.leaf_G;
     if (p->prm.g.b == 64) {
-        if (G >= 8) leaf_h16_kt<4, 8>(p, nz, st, x, psi, wstride_T, psec, wstride_cx, tb, conj);
-        else if (G >= 4) leaf_h16_kt<4, 4>(p, nz, st, x, psi, wstride_T, psec, wstride_cx, tb, conj);
-        else leaf_h16_kt<4, 2>(p, nz, st, x, psi, wstride_T, psec, wstride_cx, tb, conj);
+        if (G >= 8) leaf_h16_kt<4, 8, TAB>(p, nz, st, x, psi, wstride_T, psec, wstride_cx, tb, conj);
+        else if (G >= 4) leaf_h16_kt<4, 4, TAB>(p, nz, st, x, psi, wstride_T, psec, wstride_cx, tb, conj);
+        else leaf_h16_kt<4, 2, TAB>(p, nz, st, x, psi, wstride_T, psec, wstride_cx, tb, conj);
     } else {
-        if (G >= 8) leaf_h16_kt<2, 8>(p, nz, st, x, psi, wstride_T, psec, wstride_cx, tb, conj);
-        else if (G >= 4) leaf_h16_kt<2, 4>(p, nz, st, x, psi, wstride_T, psec, wstride_cx, tb, conj);
-        else leaf_h16_kt<2, 2>(p, nz, st, x, psi, wstride_T, psec, wstride_cx, tb, conj);
+        if (G >= 8) leaf_h16_kt<2, 8, TAB>(p, nz, st, x, psi, wstride_T, psec, wstride_cx, tb, conj);
+        else if (G >= 4) leaf_h16_kt<2, 4, TAB>(p, nz, st, x, psi, wstride_T, psec, wstride_cx, tb, conj);
+        else leaf_h16_kt<2, 2, TAB>(p, nz, st, x, psi, wstride_T, psec, wstride_cx, tb, conj);
     }
 }
 
 // The kernel's discarded stores (section 0, padding terms) go to sumsec,
 // which the section-sum reduction launched after it overwrites.
-template <typename T, int VEC, int MT>
-void s2m_mma_ks(const Geom &g, const Shapes &sh, unsigned nz, hipStream_t st, const cx<T> *x, T *phi,
+template <typename T, int VEC, int MT, bool TAB>
+void s2m_mma_ks(const Geom &g, const Shapes &sh, unsigned nz, hipStream_t st, SigAddr<const cx<T>> x, T *phi,
                 int64_t wstride_T, c64 *sums_part, int64_t wstride_c64, const T *initch, const T *opfrag,
                 c64 *sumsec, int conj) {
     const dim3 grid(sh.s2m_nblk, 1, nz);
@@ -578,57 +579,57 @@ void s2m_mma_ks(const Geom &g, const Shapes &sh, unsigned nz, hipStream_t st, co
     switch (g.b) {
         case 16:
             if (quad)
-                hipLaunchKernelGGL((k_s2m_mma<T, VEC, MT, 4, true>), grid, dim3(256), 0, st, g, x, g.N, phi,
+                hipLaunchKernelGGL((k_s2m_mma<T, VEC, MT, 4, true, TAB>), grid, dim3(256), 0, st, g, x, phi,
                                    wstride_T, sums_part, wstride_c64, initch, opfrag, sink, sink_stride, conj);
             else
-                hipLaunchKernelGGL((k_s2m_mma<T, VEC, MT, 4, false>), grid, dim3(256), 0, st, g, x, g.N, phi,
+                hipLaunchKernelGGL((k_s2m_mma<T, VEC, MT, 4, false, TAB>), grid, dim3(256), 0, st, g, x, phi,
                                    wstride_T, sums_part, wstride_c64, initch, opfrag, sink, sink_stride, conj);
             break;
         case 32:
             if (quad)
-                hipLaunchKernelGGL((k_s2m_mma<T, VEC, MT, 8, true>), grid, dim3(256), 0, st, g, x, g.N, phi,
+                hipLaunchKernelGGL((k_s2m_mma<T, VEC, MT, 8, true, TAB>), grid, dim3(256), 0, st, g, x, phi,
                                    wstride_T, sums_part, wstride_c64, initch, opfrag, sink, sink_stride, conj);
             else
-                hipLaunchKernelGGL((k_s2m_mma<T, VEC, MT, 8, false>), grid, dim3(256), 0, st, g, x, g.N, phi,
+                hipLaunchKernelGGL((k_s2m_mma<T, VEC, MT, 8, false, TAB>), grid, dim3(256), 0, st, g, x, phi,
                                    wstride_T, sums_part, wstride_c64, initch, opfrag, sink, sink_stride, conj);
             break;
         default:
             if (quad)
-                hipLaunchKernelGGL((k_s2m_mma<T, VEC, MT, 16, true>), grid, dim3(256), 0, st, g, x, g.N, phi,
+                hipLaunchKernelGGL((k_s2m_mma<T, VEC, MT, 16, true, TAB>), grid, dim3(256), 0, st, g, x, phi,
                                    wstride_T, sums_part, wstride_c64, initch, opfrag, sink, sink_stride, conj);
             else
-                hipLaunchKernelGGL((k_s2m_mma<T, VEC, MT, 16, false>), grid, dim3(256), 0, st, g, x, g.N, phi,
+                hipLaunchKernelGGL((k_s2m_mma<T, VEC, MT, 16, false, TAB>), grid, dim3(256), 0, st, g, x, phi,
                                    wstride_T, sums_part, wstride_c64, initch, opfrag, sink, sink_stride, conj);
             break;
     }
 }
 
 // matrix-core S2M (complex64): VEC from the plan's shapes, MT = 16-term blocks
-template <typename T>
-void launch_s2m_mma(const Geom &g, const Shapes &sh, unsigned nz, hipStream_t st, const cx<T> *x, T *phi,
+template <typename T, bool TAB>
+void launch_s2m_mma(const Geom &g, const Shapes &sh, unsigned nz, hipStream_t st, SigAddr<const cx<T>> x, T *phi,
                     int64_t wstride_T, c64 *sums_part, int64_t wstride_c64, const T *initch, const T *opfrag,
                     c64 *sumsec, int conj) {
     if constexpr (sizeof(T) == 4) {
         if (sh.s2m_vec == 4) {
-            if (g.terms <= 16) s2m_mma_ks<T, 4, 1>(g, sh, nz, st, x, phi, wstride_T, sums_part, wstride_c64, initch, opfrag, sumsec, conj);
-            else s2m_mma_ks<T, 4, 2>(g, sh, nz, st, x, phi, wstride_T, sums_part, wstride_c64, initch, opfrag, sumsec, conj);
+            if (g.terms <= 16) s2m_mma_ks<T, 4, 1, TAB>(g, sh, nz, st, x, phi, wstride_T, sums_part, wstride_c64, initch, opfrag, sumsec, conj);
+            else s2m_mma_ks<T, 4, 2, TAB>(g, sh, nz, st, x, phi, wstride_T, sums_part, wstride_c64, initch, opfrag, sumsec, conj);
         } else {
-            if (g.terms <= 16) s2m_mma_ks<T, 2, 1>(g, sh, nz, st, x, phi, wstride_T, sums_part, wstride_c64, initch, opfrag, sumsec, conj);
-            else s2m_mma_ks<T, 2, 2>(g, sh, nz, st, x, phi, wstride_T, sums_part, wstride_c64, initch, opfrag, sumsec, conj);
+            if (g.terms <= 16) s2m_mma_ks<T, 2, 1, TAB>(g, sh, nz, st, x, phi, wstride_T, sums_part, wstride_c64, initch, opfrag, sumsec, conj);
+            else s2m_mma_ks<T, 2, 2, TAB>(g, sh, nz, st, x, phi, wstride_T, sums_part, wstride_c64, initch, opfrag, sumsec, conj);
         }
     }
 }
 
-template <typename T, int PT, typename Mark>
-void s2m_tree_pt(const Geom &g, const Shapes &sh, unsigned nz, hipStream_t st, const cx<T> *x, T *phi, T *psi,
+template <typename T, int PT, bool TAB, typename Mark>
+void s2m_tree_pt(const Geom &g, const Shapes &sh, unsigned nz, hipStream_t st, SigAddr<const cx<T>> x, T *phi, T *psi,
                  int64_t wstride_T, c64 *sums_part, int64_t wstride_c64, c64 *sumsec, const TabPtrs<T> &tb,
                  int conj, Mark &mark) {
     if (sh.s2m_vec)
-        launch_s2m_mma<T>(g, sh, nz, st, x, phi, wstride_T, sums_part, wstride_c64, tb.initch, tb.opfrag, sumsec,
+        launch_s2m_mma<T, TAB>(g, sh, nz, st, x, phi, wstride_T, sums_part, wstride_c64, tb.initch, tb.opfrag, sumsec,
                           conj);
     else
-        hipLaunchKernelGGL((k_s2m<T, PT>), dim3(sh.s2m_nblk, 1, nz), dim3(kS2mThreads), sh.s2m_lds, st, g, x,
-                           g.N, phi, wstride_T, sums_part, wstride_c64, tb.initch, conj);
+        hipLaunchKernelGGL((k_s2m<T, PT, TAB>), dim3(sh.s2m_nblk, 1, nz), dim3(kS2mThreads), sh.s2m_lds, st, g, x,
+                           phi, wstride_T, sums_part, wstride_c64, tb.initch, conj);
     hipLaunchKernelGGL(k_sums_reduce, dim3(g.P, 1, nz), dim3(256), 0, st, g, sums_part, sh.s2m_nblk, wstride_c64,
                        sumsec, wstride_c64);
     mark(1);
@@ -725,20 +726,23 @@ void s2m_tree_pt(const Geom &g, const Shapes &sh, unsigned nz, hipStream_t st, c
     }
 }
 
-template <typename T, typename Mark>
+template <typename T, bool TAB, typename Mark>
 void launch_s2m_tree(int pt, const fmmfft_plan_s *, const Geom &g, const Shapes &sh, unsigned nz, hipStream_t st,
-                     const cx<T> *x, T *phi, T *psi, int64_t wstride_T, c64 *sums_part, int64_t wstride_c64,
+                     SigAddr<const cx<T>> x, T *phi, T *psi, int64_t wstride_T, c64 *sums_part, int64_t wstride_c64,
                      c64 *sumsec, const TabPtrs<T> &tb, int conj, Mark &mark) {
     switch (pt) {
-        case 8: s2m_tree_pt<T, 8>(g, sh, nz, st, x, phi, psi, wstride_T, sums_part, wstride_c64, sumsec, tb, conj, mark); break;
-        case 16: s2m_tree_pt<T, 16>(g, sh, nz, st, x, phi, psi, wstride_T, sums_part, wstride_c64, sumsec, tb, conj, mark); break;
-        case 24: s2m_tree_pt<T, 24>(g, sh, nz, st, x, phi, psi, wstride_T, sums_part, wstride_c64, sumsec, tb, conj, mark); break;
-        default: s2m_tree_pt<T, 32>(g, sh, nz, st, x, phi, psi, wstride_T, sums_part, wstride_c64, sumsec, tb, conj, mark); break;
+        case 8: s2m_tree_pt<T, 8, TAB>(g, sh, nz, st, x, phi, psi, wstride_T, sums_part, wstride_c64, sumsec, tb, conj, mark); break;
+        case 16: s2m_tree_pt<T, 16, TAB>(g, sh, nz, st, x, phi, psi, wstride_T, sums_part, wstride_c64, sumsec, tb, conj, mark); break;
+        case 24: s2m_tree_pt<T, 24, TAB>(g, sh, nz, st, x, phi, psi, wstride_T, sums_part, wstride_c64, sumsec, tb, conj, mark); break;
+        default: s2m_tree_pt<T, 32, TAB>(g, sh, nz, st, x, phi, psi, wstride_T, sums_part, wstride_c64, sumsec, tb, conj, mark); break;
     }
 }
 
-template <typename T>
-int run(const fmmfft_plan_s *p, int64_t count, const void *d_in, void *d_out, int dir, char *ws,
+// One execute of count signals: their inputs and outputs at x.at(sig) /
+// out.at(sig) (contiguous, or a device table of per-signal addresses),
+// workspace ws (count strides).
+template <typename T, bool TAB>
+int run(const fmmfft_plan_s *p, int64_t count, SigAddr<const cx<T>> x, SigAddr<cx<T>> out, int dir, char *ws,
         hipStream_t st) {
     const Geom &g = p->prm.g;
     const Shapes &sh = p->sh;
@@ -750,13 +754,16 @@ int run(const fmmfft_plan_s *p, int64_t count, const void *d_in, void *d_out, in
     const int64_t wstride_c64 = (int64_t)(wl.stride / sizeof(c64));
     const int64_t wstride_cx = (int64_t)(wl.stride / sizeof(cx<T>));
 
-    const cx<T> *x = static_cast<const cx<T> *>(d_in);
-    cx<T> *out = static_cast<cx<T> *>(d_out);
+    // the same output buffers as FFT input, and the workspace rows as
+    // signal addresses
+    const SigAddr<const cx<T>> out_in{out.base, out.stride, out.tab};
     T *phi = reinterpret_cast<T *>(ws + wl.phi);
     T *psi = reinterpret_cast<T *>(ws + wl.psi);
     c64 *sums_part = reinterpret_cast<c64 *>(ws + wl.sums_part);
     c64 *sumsec = reinterpret_cast<c64 *>(ws + wl.sumsec);
     cx<T> *psec = reinterpret_cast<cx<T> *>(ws + wl.psec);
+    const SigAddr<cx<T>> psec_a{psec, wstride_cx, nullptr};
+    const SigAddr<const cx<T>> psec_in{psec, wstride_cx, nullptr};
     const c64 *gfac = reinterpret_cast<const c64 *>(p->tab64 + p->tl.gfac);
     const c64 *wP = reinterpret_cast<const c64 *>(p->tab64 + p->tl.wP);
     const c64 *tlo = reinterpret_cast<const c64 *>(p->tab64 + p->tl.tlo);
@@ -771,26 +778,26 @@ int run(const fmmfft_plan_s *p, int64_t count, const void *d_in, void *d_out, in
         if (ev) (void)hipEventRecord(ev[i], st);
     };
     mark(0);
-    launch_s2m_tree<T>(padded_terms(g.terms), p, g, sh, nz, st, x, phi, psi, wstride_T, sums_part, wstride_c64,
+    launch_s2m_tree<T, TAB>(padded_terms(g.terms), p, g, sh, nz, st, x, phi, psi, wstride_T, sums_part, wstride_c64,
                        sumsec, tb, conj, mark);
     mark(3);
     bool leaf_done = false;
     if constexpr (sizeof(T) == 4) {
         if (sh.leaf_h16) {
-            launch_leaf_h16(p, nz, st, x, psi, wstride_T, psec, wstride_cx, tb, conj);
+            launch_leaf_h16<TAB>(p, nz, st, x, psi, wstride_T, psec, wstride_cx, tb, conj);
             leaf_done = true;
         }
     }
     if (leaf_done) {
     } else if (sh.leaf_mma) {
-        launch_leaf_mma<T>(g, sh, nz, st, x, psi, wstride_T, psec, wstride_cx, tb, conj);
+        launch_leaf_mma<T, TAB>(g, sh, nz, st, x, psi, wstride_T, psec, wstride_cx, tb, conj);
     } else {
-        hipLaunchKernelGGL(k_leaf<T>, dim3(g.s / sh.leaf_bb, g.P / sh.leaf_G, nz), dim3(256), sh.leaf_lds, st, g,
-                           x, g.N, psi, wstride_T, psec, wstride_cx, tb.evalm, tb.evalmh, tb.kv, sh.leaf_G,
+        hipLaunchKernelGGL((k_leaf<T, TAB>), dim3(g.s / sh.leaf_bb, g.P / sh.leaf_G, nz), dim3(256), sh.leaf_lds, st, g,
+                           x, psi, wstride_T, psec, wstride_cx, tb.evalm, tb.evalmh, tb.kv, sh.leaf_G,
                            sh.leaf_bb, conj, sh.rm);
     }
     mark(4);
-    launch_combine<T>(g, nz, st, psec, wstride_cx, sumsec, wstride_c64, gfac, wP, out, g.N);
+    launch_combine<T, TAB>(g, nz, st, psec, wstride_cx, sumsec, wstride_c64, gfac, wP, out);
     mark(5);
 
     const FftLayout &f = p->fft;
@@ -803,7 +810,7 @@ int run(const fmmfft_plan_s *p, int64_t count, const void *d_in, void *d_out, in
         fp.out_fs = g.M; fp.out_ks = 1;
         fp.conj_out = conj;
         fp.logR = f.logR;
-        launch_fft<T>(f.M1, g, fp, nz, st, out, out, g.N, g.N, twA, tlo, thi);
+        launch_fft<T, TAB>(f.M1, g, fp, nz, st, out_in, out, twA, tlo, thi);
         mark(6);
     } else {
         // first pass: sub-FFTs of length M1 along the columns of [M1][R]
@@ -822,7 +829,7 @@ int run(const fmmfft_plan_s *p, int64_t count, const void *d_in, void *d_out, in
         a.out_fs = 1; a.out_ks = R;
         a.twiddle = 1;
         a.logR = f.logR;
-        launch_fft<T>(f.M1, g, a, nz, st, out, psec, g.N, wstride_cx, twA, tlo, thi);
+        launch_fft<T, TAB>(f.M1, g, a, nz, st, out_in, psec_a, twA, tlo, thi);
         mark(6);
         if (f.passes == 2) {
             FftPass bq{};
@@ -831,7 +838,7 @@ int run(const fmmfft_plan_s *p, int64_t count, const void *d_in, void *d_out, in
             bq.out_fs = 1; bq.out_ks = f.M1;
             bq.conj_out = conj;
             bq.logR = f.logR;
-            launch_fft<T>(f.M2, g, bq, nz, st, psec, out, wstride_cx, g.N, twB, tlo, thi);
+            launch_fft<T, TAB>(f.M2, g, bq, nz, st, psec_in, out, twB, tlo, thi);
         } else {
             // Z[k1][r], r = n2 M3 + n3.  Second pass, in place: for every
             // (k1, n3) (sub-FFT f = k1 M3 + n3, a column of row k1) the
@@ -844,7 +851,7 @@ int run(const fmmfft_plan_s *p, int64_t count, const void *d_in, void *d_out, in
             b2.out_fs = 1; b2.out_ks = f.M3; b2.out_bs = R; b2.out_tile = l3;
             b2.twiddle = 1; b2.tw_lf = l3; b2.tw_shift = l1;
             b2.logR = f.logR;
-            launch_fft<T>(f.M2, g, b2, nz, st, psec, psec, wstride_cx, wstride_cx, twB, tlo, thi);
+            launch_fft<T, TAB>(f.M2, g, b2, nz, st, psec_in, psec_a, twB, tlo, thi);
             // third pass: for every (k1, k2) (sub-FFT f = k2 M1 + k1: k1
             // fastest, so that the outputs X[k1 + M1 k2 + M1 M2 k3] of
             // neighbouring sub-FFTs are neighbours) the M3-point FFT of the
@@ -855,7 +862,7 @@ int run(const fmmfft_plan_s *p, int64_t count, const void *d_in, void *d_out, in
             b3.out_fs = 1; b3.out_ks = f.M1 * f.M2;
             b3.conj_out = conj;
             b3.logR = f.logR;
-            launch_fft<T>(f.M3, g, b3, nz, st, psec, out, wstride_cx, g.N, twC, tlo, thi);
+            launch_fft<T, TAB>(f.M3, g, b3, nz, st, psec_in, out, twC, tlo, thi);
         }
     }
     mark(7);
@@ -1107,9 +1114,32 @@ int fmmfft_execute_ws(fmmfft_plan plan, int64_t count, const void *d_in, void *d
     // the S2M kernel streams the input with 16-byte loads
     if (((uintptr_t)d_in % 16) != 0 || ((uintptr_t)d_out % (2 * plan->prm.esize)) != 0) return FMMFFT_EINVAL;
     hipStream_t st = static_cast<hipStream_t>(stream);
-    if (is_c64(plan->prm.dtype)) return run<float>(plan, count, d_in, d_out, dir, (char *)d_work, st);
-    return run<double>(plan, count, d_in, d_out, dir, (char *)d_work, st);
+    const int64_t n = plan->prm.g.N;
+    if (is_c64(plan->prm.dtype))
+        return run<float, false>(plan, count, SigAddr<const c32>{static_cast<const c32 *>(d_in), n, nullptr},
+                                 SigAddr<c32>{static_cast<c32 *>(d_out), n, nullptr}, dir, (char *)d_work, st);
+    return run<double, false>(plan, count, SigAddr<const c64>{static_cast<const c64 *>(d_in), n, nullptr},
+                              SigAddr<c64>{static_cast<c64 *>(d_out), n, nullptr}, dir, (char *)d_work, st);
 }
+
+}  // extern "C"
+
+int fmmfft::execute_table(fmmfft_plan plan, int64_t count, const void *const *d_in_tab, void *const *d_out_tab,
+                          int32_t dir, void *d_work, size_t work_bytes, void *stream) {
+    if (!plan || !d_in_tab || !d_out_tab || !d_work || count <= 0 || count > 65535) return FMMFFT_EINVAL;
+    if (dir != FMMFFT_FORWARD && dir != FMMFFT_INVERSE) return FMMFFT_EINVAL;
+    if (work_bytes < plan->ws.stride * (size_t)count || ((uintptr_t)d_work % kAlign) != 0) return FMMFFT_EINVAL;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    if (is_c64(plan->prm.dtype))
+        return run<float, true>(plan, count, SigAddr<const c32>{nullptr, 0, reinterpret_cast<const c32 *const *>(d_in_tab)},
+                                SigAddr<c32>{nullptr, 0, reinterpret_cast<c32 *const *>(d_out_tab)}, dir,
+                                (char *)d_work, st);
+    return run<double, true>(plan, count, SigAddr<const c64>{nullptr, 0, reinterpret_cast<const c64 *const *>(d_in_tab)},
+                             SigAddr<c64>{nullptr, 0, reinterpret_cast<c64 *const *>(d_out_tab)}, dir,
+                             (char *)d_work, st);
+}
+
+extern "C" {
 
 int fmmfft_execute_batched(fmmfft_plan plan, int64_t count, const void *d_in, void *d_out, int32_t dir,
                            void *stream) {

@@ -13,4 +13,14 @@ __attribute__((visibility("hidden"))) int plan_create_impl(fmmfft_plan *out, int
                                                            int32_t terms, int32_t leaf, int32_t dtype,
                                                            int32_t max_batch, bool with_workspace);
 
+// fmmfft_execute_ws over count separate signals: signal i reads
+// d_in_tab[i] and writes d_out_tab[i], both device tables of device
+// addresses (the caller checks each address's alignment, as
+// fmmfft_execute_ws does for its two pointers).  Tables and workspace must
+// stay untouched until the execute has run on the stream.
+__attribute__((visibility("hidden"))) int execute_table(fmmfft_plan plan, int64_t count,
+                                                        const void *const *d_in_tab, void *const *d_out_tab,
+                                                        int32_t dir, void *d_work, size_t work_bytes,
+                                                        void *stream);
+
 }  // namespace fmmfft

@@ -220,9 +220,9 @@ constexpr int kFftThreads = 512;
 enum FftForm { FFT_SINGLE = 0, FFT_PASS1 = 1, FFT_PASS2 = 2 };
 
 // PERSIST = false: one tile per workgroup (grid = tiles), no prefetch.
-template <typename T, int L, int FORM, int C = fft_cols<T, L>(), bool PERSIST = true>
-__global__ __launch_bounds__(kFftThreads) void k_fft(Geom g, FftPass fp, const cx<T> *__restrict__ in,
-                                             cx<T> *__restrict__ out, int64_t in_stride, int64_t out_stride,
+template <typename T, int L, int FORM, int C = fft_cols<T, L>(), bool PERSIST = true, bool TAB = false>
+__global__ __launch_bounds__(kFftThreads) void k_fft(Geom g, FftPass fp, SigAddr<const cx<T>> in,
+                                             SigAddr<cx<T>> out,
                                              const cx<T> *__restrict__ twL, const c64 *__restrict__ tlo,
                                              const c64 *__restrict__ thi, int nsig) {
     constexpr int PSX = sizeof(T) == 8 ? 3 : 4;  // FP64 stages are radix <= 8
@@ -253,7 +253,7 @@ __global__ __launch_bounds__(kFftThreads) void k_fft(Geom g, FftPass fp, const c
     T vr[PER], vi[PER];
     auto prefetch = [&](int tt) __attribute__((always_inline)) {
         const int sig = tt / ntiles, t = tt - sig * ntiles;
-        const cx<T> *in_s = in + (int64_t)sig * in_stride;
+        const cx<T> *in_s = in.template at<TAB>(sig);
         const int fg0 = t * C;
         const int cl = min(C, nf - fg0) - 1;
         const int k2 = fg0 / F, f0 = fg0 - k2 * F, base = k2 * M;
@@ -280,7 +280,7 @@ __global__ __launch_bounds__(kFftThreads) void k_fft(Geom g, FftPass fp, const c
         }
         __syncthreads();
         const int sig = tt / ntiles, t = tt - sig * ntiles;
-        cx<T> *out_s = out + (int64_t)sig * out_stride;
+        cx<T> *out_s = out.template at<TAB>(sig);
         const int fg0 = t * C;
         const int cv = min(C, nf - fg0);
         const int k2 = fg0 / F, f0 = fg0 - k2 * F, base = k2 * M;

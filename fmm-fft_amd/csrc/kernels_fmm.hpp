@@ -38,8 +38,8 @@ namespace fmmfft {
 // ---------------------------------------------------------------------------
 constexpr int kS2mThreads = 512;
 
-template <typename T, int PT>
-__global__ __launch_bounds__(kS2mThreads) void k_s2m(Geom g, const cx<T> *__restrict__ x, int64_t x_stride,
+template <typename T, int PT, bool TAB = false>
+__global__ __launch_bounds__(kS2mThreads) void k_s2m(Geom g, SigAddr<const cx<T>> xa,
                                                      T *__restrict__ phi, int64_t tree_stride,
                                                      c64 *__restrict__ sums_part, int64_t sums_stride,
                                                      const T *__restrict__ initch, int conj) {
@@ -58,7 +58,7 @@ __global__ __launch_bounds__(kS2mThreads) void k_s2m(Geom g, const cx<T> *__rest
     const int tile_elems = bx_n * b * P, tile_vecs = tile_elems * (int)sizeof(cx<T>) / 16;
     const int ntiles = g.s / bx_n;
     const int sig = blockIdx.z;
-    x += (int64_t)sig * x_stride;
+    const cx<T> *__restrict__ x = xa.template at<TAB>(sig);
     phi += (int64_t)sig * tree_stride + level_off(g, g.n);
     sums_part += (int64_t)sig * sums_stride + (int64_t)blockIdx.x * P;
     for (int i = threadIdx.x; i < b * PT; i += NT) {
@@ -196,8 +196,8 @@ __global__ __launch_bounds__(256) void k_sums_reduce(Geom g, const c64 *__restri
 // four consecutive targets of one (section, box) and slides the Toeplitz
 // kernel through registers: one x load and one kernel load per 4 complex MACs.
 // ---------------------------------------------------------------------------
-template <typename T>
-__global__ __launch_bounds__(256) void k_leaf(Geom g, const cx<T> *__restrict__ x, int64_t x_stride,
+template <typename T, bool TAB = false>
+__global__ __launch_bounds__(256) void k_leaf(Geom g, SigAddr<const cx<T>> xa,
                                               const T *__restrict__ psi, int64_t tree_stride,
                                               cx<T> *__restrict__ psec, int64_t psec_stride,
                                               const T *__restrict__ evalm, const T *__restrict__ evalmh,
@@ -211,7 +211,7 @@ __global__ __launch_bounds__(256) void k_leaf(Geom g, const cx<T> *__restrict__ 
     T *kvs = reinterpret_cast<T *>(xw + (size_t)rows * G);        // [G][kvl]
     T *loc = kvs + (size_t)G * kvl;                               // [bb+1][G][2][terms]
     const int sig = blockIdx.z;
-    x += (int64_t)sig * x_stride;
+    const cx<T> *__restrict__ x = xa.template at<TAB>(sig);
     psi += (int64_t)sig * tree_stride + level_off(g, g.n);
     psec += (int64_t)sig * psec_stride;
     const int i0 = blockIdx.x * bb;
@@ -409,8 +409,8 @@ template <> struct VecOf<float, 2> { typedef float2 type; };
 __device__ inline float velem(const float4 &v, int e) { return e == 0 ? v.x : e == 1 ? v.y : e == 2 ? v.z : v.w; }
 __device__ inline float velem(const float2 &v, int e) { return e == 0 ? v.x : v.y; }
 
-template <typename T, int VEC, int MT, int KS, bool QUAD>
-__global__ __launch_bounds__(256) void k_s2m_mma(Geom g, const cx<T> *__restrict__ x, int64_t x_stride,
+template <typename T, int VEC, int MT, int KS, bool QUAD, bool TAB = false>
+__global__ __launch_bounds__(256) void k_s2m_mma(Geom g, SigAddr<const cx<T>> xa,
                                                  T *__restrict__ phi, int64_t tree_stride,
                                                  c64 *__restrict__ sums_part, int64_t sums_stride,
                                                  const T *__restrict__ initch, const T *__restrict__ opfrag,
@@ -424,7 +424,7 @@ __global__ __launch_bounds__(256) void k_s2m_mma(Geom g, const cx<T> *__restrict
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, col = lane & 15, kr = lane >> 4;
     const int chunk = wave % nchunk, pslot = wave / nchunk;
     const int sig = blockIdx.z;
-    const T *xr = reinterpret_cast<const T *>(x + (int64_t)sig * x_stride);
+    const T *xr = reinterpret_cast<const T *>(xa.template at<TAB>(sig));
     T *phi_n = phi + (int64_t)sig * tree_stride + level_off(g, g.n);      // leaves
     T *phi_p = phi + (int64_t)sig * tree_stride + level_off(g, g.n - 1);  // their parents
     sums_part += (int64_t)sig * sums_stride + (int64_t)blockIdx.x * P;
@@ -618,9 +618,9 @@ constexpr int leaf_waves() {
     return 1;
 }
 
-template <typename T, int NRT, int G, int KT, int CT>
+template <typename T, int NRT, int G, int KT, int CT, bool TAB = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(leaf_waves<T, NRT, G, KT, CT>())))
-void k_leaf_mma(Geom g, const cx<T> *__restrict__ x, int64_t x_stride,
+void k_leaf_mma(Geom g, SigAddr<const cx<T>> xa,
                                                   const T *__restrict__ psi, int64_t tree_stride,
                                                   cx<T> *__restrict__ psec, int64_t psec_stride,
                                                   const T *__restrict__ evfrag, const T *__restrict__ evalm,
@@ -714,7 +714,7 @@ void k_leaf_mma(Geom g, const cx<T> *__restrict__ x, int64_t x_stride,
     // counter wait after a branch covers every load in flight.
     auto prefetch = [&](int tile) __attribute__((always_inline)) {
         const int sig = tile / ntiles, lid = tile - sig * ntiles;
-        const cx<T> *xs = x + (int64_t)sig * x_stride;
+        const cx<T> *xs = xa.template at<TAB>(sig);
         const T *ps = psi + (int64_t)sig * tree_stride;
         const int i0 = (lid / ngroups) * BB;
         const int mu_w0 = (i0 - 2) * b;
@@ -945,9 +945,9 @@ struct LeafH16 {
     static constexpr int NF = NKS + NRT - 1;                             // Toeplitz fragments
 };
 
-template <int NRT, int G, int KT>
+template <int NRT, int G, int KT, bool TAB = false>
 __global__ __launch_bounds__(64 * (G > 4 ? G : 4)) __attribute__((amdgpu_waves_per_eu(2)))
-void k_leaf_h16(Geom g, const c32 *__restrict__ x, int64_t x_stride, const float *__restrict__ psi,
+void k_leaf_h16(Geom g, SigAddr<const c32> xa, const float *__restrict__ psi,
                 int64_t tree_stride, c32 *__restrict__ psec, int64_t psec_stride,
                 const float *__restrict__ evfrag, const float *__restrict__ evalm,
                 const float *__restrict__ evalmh, const float *__restrict__ kv,
@@ -1028,7 +1028,7 @@ void k_leaf_h16(Geom g, const c32 *__restrict__ x, int64_t x_stride, const float
     float lr[LPER];
     auto prefetch = [&](int tile) __attribute__((always_inline)) {
         const int sig = tile / ntiles, lid = tile - sig * ntiles;
-        const c32 *xs = x + (int64_t)sig * x_stride;
+        const c32 *xs = xa.template at<TAB>(sig);
         const float *ps = psi + (int64_t)sig * tree_stride;
         const int i0 = (lid / ngroups) * BB;
         const int mu_w0 = (i0 - 2) * b;
@@ -1711,11 +1711,11 @@ constexpr int kCombineThreads = 256;
 template <int P>
 constexpr int combine_rows() { return P >= 32 ? 2048 / P : 1024 / P; }
 
-template <typename T, int P>
+template <typename T, int P, bool TAB = false>
 __global__ __launch_bounds__(kCombineThreads) void k_combine(Geom g, const cx<T> *__restrict__ psec, int64_t psec_stride,
                                                  const c64 *__restrict__ sumsec, int64_t sumsec_stride,
                                                  const c64 *__restrict__ gfac, const c64 *__restrict__ wP,
-                                                 cx<T> *__restrict__ y, int64_t y_stride) {
+                                                 SigAddr<cx<T>> ya) {
     constexpr int RB = combine_rows<P>();
     constexpr int NT = kCombineThreads;
     // one pad slot per 8 elements (an 8-way bank conflict in the first
@@ -1726,7 +1726,7 @@ __global__ __launch_bounds__(kCombineThreads) void k_combine(Geom g, const cx<T>
     const int sig = blockIdx.z;
     psec += (int64_t)sig * psec_stride;
     sumsec += (int64_t)sig * sumsec_stride;
-    y += (int64_t)sig * y_stride;
+    cx<T> *__restrict__ y = ya.template at<TAB>(sig);
     const int64_t ntiles = (g.M + RB - 1) / RB;
     const int64_t nwg = gridDim.x;
     const int64_t xo = xcd_remap(blockIdx.x, nwg);

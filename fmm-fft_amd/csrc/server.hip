@@ -13,15 +13,18 @@
 //
 // A request carries caller-owned device buffers and the caller's stream: an
 // event recorded on that stream at submit time is waited for on the slot
-// stream before the request's input is read.  Shapes with N <= kSlotElems
-// are gathered into a contiguous slot (one device-to-device copy per
-// request), executed as one batched launch sequence captured once per
-// (slot, count) as a hipGraph, and scattered back; larger shapes execute
-// directly on the caller's buffers.  Each shape owns two slots (two
-// workspaces), so one batch is filled and copied while the previous one
-// computes, and batches of different shapes run on different streams and
-// overlap on the device.  The scheduler launches the oldest pending request
-// whose shape has a free slot (a busy shape does not hold back the others).
+// stream before the request's input is read.  A batch of same-shape
+// requests (up to kBatchElems points in all) runs in place on the callers'
+// buffers: the slot's device table of per-request input and output
+// addresses is refreshed from pinned host memory by one small copy, and the
+// batched launch sequence, which reads every signal's addresses from that
+// table, is captured once per (slot, count) as a hipGraph and replayed.
+// Each shape owns two slots (two workspaces and tables), so one batch is
+// set up while the previous one computes, and batches of different shapes
+// run on different streams and overlap on the device, up to
+// kMaxBatchesInFlight batches at once.  The scheduler launches the oldest
+// pending request whose shape has a free slot (a busy shape does not hold
+// back the others).
 // Completion times come from a timing event recorded after each batch,
 // measured against a reference event whose host time is known, so they do
 // not depend on when the scheduler notices them.
@@ -48,9 +51,15 @@
 
 namespace {
 
-constexpr int64_t kSlotElems = int64_t(1) << 20;   // gather/graph path up to this N
-constexpr int64_t kBatchElems = int64_t(1) << 22;  // elements per gathered batch
+constexpr int64_t kBatchElems = int64_t(1) << 23;  // points per batch (all requests)
 constexpr int kSlots = 2;
+// Batches on the device at once, all shapes together.  More only queue
+// behind the few hardware queues, while a request that waits in the
+// server's queue instead joins its shape's next batch: with this cap the
+// mixed stream of 52 shapes ran 9.5k req/s at saturation against 8.5k with
+// every shape's two slots in use (3.6k against 7.6k launches per 10k
+// requests).
+constexpr int kMaxBatchesInFlight = 16;
 
 double now_s() {
     using namespace std::chrono;
@@ -80,8 +89,11 @@ struct Req {
 struct Slot {
     hipStream_t st = nullptr;
     hipEvent_t ev = nullptr;  // timing event after the batch's last operation
-    void *ws = nullptr, *buf_in = nullptr, *buf_out = nullptr;
+    void *ws = nullptr;
     size_t ws_bytes = 0;
+    // per-request addresses, inputs [0, maxb) then outputs [maxb, 2 maxb):
+    // the device table the kernels read and its pinned host staging
+    void **tab = nullptr, **htab = nullptr;
     bool busy = false;
     std::vector<std::shared_ptr<Req>> batch;
     std::map<int, hipGraphExec_t> graphs;  // batch count -> executable graph
@@ -91,7 +103,6 @@ struct Shape {
     Key key;
     fmmfft_plan plan = nullptr;
     int maxb = 1;
-    bool gathered = false;
     size_t esize = 8, bytes = 0;
     Slot slots[kSlots];
     double last_used = 0;
@@ -118,6 +129,7 @@ struct fmmfft_server_s {
     uint64_t next_id = 1;
     bool stop = false;
     int64_t inflight = 0;
+    int busy_slots = 0;  // batches on the device (scheduler thread only)
     std::vector<double> lat_ms;
     double first_submit = 0, last_done = 0, gflop = 0;
     int64_t batches = 0, plans_created = 0, plans_evicted = 0, graphs_captured = 0;
@@ -138,16 +150,13 @@ struct fmmfft_server_s {
 
 namespace {
 
-// device memory the server holds for a shape: tables, and per slot the
-// workspace for maxb signals plus (gathered shapes) the in/out buffers
-size_t shape_bytes(const Key &k, int maxb, bool gathered, size_t *ws_bytes) {
+// device memory the server holds for a shape: the plan's tables, and per
+// slot the workspace for maxb signals and the address table
+size_t shape_bytes(const Key &k, int maxb, size_t *ws_bytes) {
     size_t tb = 0, wb = 0;
     if (fmmfft_plan_sizes(k.n, k.p, k.terms, k.leaf, k.dtype, &tb, &wb) != FMMFFT_OK) return 0;
-    const size_t esize = k.dtype == FMMFFT_C128 ? 16 : 8;
     *ws_bytes = wb * (size_t)maxb;
-    size_t per_slot = *ws_bytes;
-    if (gathered) per_slot += 2 * (size_t)maxb * (size_t)k.n * esize;
-    return tb + kSlots * per_slot;
+    return tb + kSlots * (*ws_bytes + 2 * sizeof(void *) * (size_t)maxb);
 }
 
 }  // namespace
@@ -157,8 +166,8 @@ void fmmfft_server_s::free_shape(Shape &sh) {
         for (auto &g : sl.graphs) (void)hipGraphExecDestroy(g.second);
         sl.graphs.clear();
         if (sl.ws) (void)hipFree(sl.ws);
-        if (sl.buf_in) (void)hipFree(sl.buf_in);
-        if (sl.buf_out) (void)hipFree(sl.buf_out);
+        if (sl.tab) (void)hipFree(sl.tab);
+        if (sl.htab) (void)hipHostFree(sl.htab);
         if (sl.ev) (void)hipEventDestroy(sl.ev);
         if (sl.st) (void)hipStreamDestroy(sl.st);
         sl = Slot{};
@@ -185,20 +194,19 @@ bool fmmfft_server_s::evict_one(const Shape *keep) {
     return true;
 }
 
-// Plan (tables only) + workspaces (+ gather buffers) for a shape, created
+// Plan (tables only) + workspaces and address tables for a shape, created
 // under the budget.  err: FMMFFT_OK, FMMFFT_ENOMEM / EINVAL / ENOTSUP (cannot
 // run even alone: reject) or -1 (does not fit now: wait for completions).
 Shape *fmmfft_server_s::get_shape(const Key &k, int &err) {
     err = FMMFFT_OK;
     auto it = shapes.find(k);
     if (it != shapes.end()) return it->second.get();
-    const bool gathered = k.n <= kSlotElems;
-    int maxb = gathered ? (int)std::max<int64_t>(1, std::min<int64_t>(max_batch, kBatchElems / k.n)) : 1;
+    int maxb = (int)std::max<int64_t>(1, std::min<int64_t>(max_batch, kBatchElems / k.n));
     size_t wsb = 0;
-    size_t need = shape_bytes(k, maxb, gathered, &wsb);
+    size_t need = shape_bytes(k, maxb, &wsb);
     while (need > budget && maxb > 1) {
         maxb /= 2;
-        need = shape_bytes(k, maxb, gathered, &wsb);
+        need = shape_bytes(k, maxb, &wsb);
     }
     if (need == 0 || need > budget) {
         err = FMMFFT_ENOMEM;
@@ -212,7 +220,6 @@ Shape *fmmfft_server_s::get_shape(const Key &k, int &err) {
     auto sh = std::make_unique<Shape>();
     sh->key = k;
     sh->maxb = maxb;
-    sh->gathered = gathered;
     sh->esize = k.dtype == FMMFFT_C128 ? 16 : 8;
     // tables only: every execute passes a slot workspace
     int rc = fmmfft::plan_create_impl(&sh->plan, k.n, k.p, k.terms, k.leaf, k.dtype, maxb, false);
@@ -221,11 +228,9 @@ Shape *fmmfft_server_s::get_shape(const Key &k, int &err) {
         if (!ok) break;
         sl.ws_bytes = wsb;
         ok = hipStreamCreateWithFlags(&sl.st, hipStreamNonBlocking) == hipSuccess &&
-             hipEventCreate(&sl.ev) == hipSuccess && hipMalloc(&sl.ws, wsb) == hipSuccess;
-        if (ok && gathered) {
-            const size_t bb = (size_t)maxb * (size_t)k.n * sh->esize;
-            ok = hipMalloc(&sl.buf_in, bb) == hipSuccess && hipMalloc(&sl.buf_out, bb) == hipSuccess;
-        }
+             hipEventCreate(&sl.ev) == hipSuccess && hipMalloc(&sl.ws, wsb) == hipSuccess &&
+             hipMalloc((void **)&sl.tab, 2 * sizeof(void *) * (size_t)maxb) == hipSuccess &&
+             hipHostMalloc((void **)&sl.htab, 2 * sizeof(void *) * (size_t)maxb, hipHostMallocDefault) == hipSuccess;
     }
     sh->bytes = need;
     used += need;
@@ -282,6 +287,7 @@ bool fmmfft_server_s::poll_completions() {
             auto batch = std::move(sl.batch);
             sl.batch.clear();
             sl.busy = false;
+            --busy_slots;
             kv.second->last_used = now_s();
             finish(batch, FMMFFT_OK);
             any = true;
@@ -298,6 +304,7 @@ bool fmmfft_server_s::launch_one() {
     bool have = false;
     {
         std::lock_guard<std::mutex> g(mu);
+        if (busy_slots >= kMaxBatchesInFlight) return false;
         std::vector<Key> busy;  // shapes seen without a free slot
         for (auto it = pending.begin(); it != pending.end() && !have; ++it) {
             const Key &k = (*it)->key;
@@ -355,47 +362,43 @@ bool fmmfft_server_s::launch_one() {
     }
     Slot *sl = sh->free_slot();
     const int cnt = (int)batch.size();
-    const size_t sig_bytes = (size_t)key.n * sh->esize;
     int rc = FMMFFT_OK;
     // inputs are complete on the callers' streams
     for (auto &r : batch)
         if (r->ready && hipStreamWaitEvent(sl->st, r->ready, 0) != hipSuccess) rc = FMMFFT_EHIP;
-    if (rc == FMMFFT_OK && sh->gathered) {
-        for (int i = 0; i < cnt && rc == FMMFFT_OK; ++i)
-            if (hipMemcpyAsync((char *)sl->buf_in + i * sig_bytes, batch[i]->in, sig_bytes, hipMemcpyDeviceToDevice,
-                               sl->st) != hipSuccess)
-                rc = FMMFFT_EHIP;
-        auto g = sl->graphs.find(cnt);
-        if (rc == FMMFFT_OK && g == sl->graphs.end()) {
-            hipGraph_t graph = nullptr;
-            hipGraphExec_t exec = nullptr;
-            if (hipStreamBeginCapture(sl->st, hipStreamCaptureModeThreadLocal) == hipSuccess) {
-                int erc = fmmfft_execute_ws(sh->plan, cnt, sl->buf_in, sl->buf_out, key.dir, sl->ws,
+    // the batch's addresses into the slot's table (the pinned staging is
+    // rewritten only once the slot's previous batch, and so its copy, ended)
+    for (int i = 0; i < cnt; ++i) {
+        sl->htab[i] = const_cast<void *>(batch[i]->in);
+        sl->htab[sh->maxb + i] = batch[i]->out;
+    }
+    if (rc == FMMFFT_OK && hipMemcpyAsync(sl->tab, sl->htab, 2 * sizeof(void *) * (size_t)sh->maxb,
+                                          hipMemcpyHostToDevice, sl->st) != hipSuccess)
+        rc = FMMFFT_EHIP;
+    auto gr = sl->graphs.find(cnt);
+    if (rc == FMMFFT_OK && gr == sl->graphs.end()) {
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        if (hipStreamBeginCapture(sl->st, hipStreamCaptureModeThreadLocal) == hipSuccess) {
+            int erc = fmmfft::execute_table(sh->plan, cnt, sl->tab, sl->tab + sh->maxb, key.dir, sl->ws,
                                             sl->ws_bytes, sl->st);
-                hipError_t ce = hipStreamEndCapture(sl->st, &graph);
-                if (erc == FMMFFT_OK && ce == hipSuccess &&
-                    hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) == hipSuccess) {
-                    sl->graphs[cnt] = exec;
-                    std::lock_guard<std::mutex> lk(mu);
-                    ++graphs_captured;
-                } else {
-                    rc = FMMFFT_EHIP;
-                }
-                if (graph) (void)hipGraphDestroy(graph);
+            hipError_t ce = hipStreamEndCapture(sl->st, &graph);
+            if (erc == FMMFFT_OK && ce == hipSuccess &&
+                hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) == hipSuccess) {
+                sl->graphs[cnt] = exec;
+                std::lock_guard<std::mutex> lk(mu);
+                ++graphs_captured;
             } else {
                 rc = FMMFFT_EHIP;
             }
-            g = sl->graphs.find(cnt);
-        }
-        if (rc == FMMFFT_OK && g != sl->graphs.end() && hipGraphLaunch(g->second, sl->st) != hipSuccess)
+            if (graph) (void)hipGraphDestroy(graph);
+        } else {
             rc = FMMFFT_EHIP;
-        for (int i = 0; i < cnt && rc == FMMFFT_OK; ++i)
-            if (hipMemcpyAsync(batch[i]->out, (char *)sl->buf_out + i * sig_bytes, sig_bytes,
-                               hipMemcpyDeviceToDevice, sl->st) != hipSuccess)
-                rc = FMMFFT_EHIP;
-    } else if (rc == FMMFFT_OK) {
-        rc = fmmfft_execute_ws(sh->plan, 1, batch[0]->in, batch[0]->out, key.dir, sl->ws, sl->ws_bytes, sl->st);
+        }
+        gr = sl->graphs.find(cnt);
     }
+    if (rc == FMMFFT_OK && gr != sl->graphs.end() && hipGraphLaunch(gr->second, sl->st) != hipSuccess)
+        rc = FMMFFT_EHIP;
     if (rc == FMMFFT_OK && hipEventRecord(sl->ev, sl->st) != hipSuccess) rc = FMMFFT_EHIP;
     if (rc != FMMFFT_OK) {
         (void)hipStreamSynchronize(sl->st);
@@ -404,8 +407,9 @@ bool fmmfft_server_s::launch_one() {
     }
     sl->batch = std::move(batch);
     sl->busy = true;
+    ++busy_slots;
     sh->last_used = now_s();
-    std::lock_guard<std::mutex> g(mu);
+    std::lock_guard<std::mutex> lk(mu);
     ++batches;
     return true;
 }
@@ -458,6 +462,9 @@ int fmmfft_server_submit(fmmfft_server srv, int64_t n, int32_t p_blocks, int32_t
     if (dir != FMMFFT_FORWARD && dir != FMMFFT_INVERSE) return FMMFFT_EINVAL;
     int rc = fmmfft_plan_sizes(n, p_blocks, terms, leaf, dtype, nullptr, nullptr);
     if (rc != FMMFFT_OK) return rc;
+    // the alignment fmmfft_execute_ws asks of its buffers
+    const uintptr_t elem = dtype == FMMFFT_C128 ? 16 : 8;  // bytes of one complex point
+    if ((uintptr_t)d_in % 16 != 0 || (uintptr_t)d_out % elem != 0) return FMMFFT_EINVAL;
     auto r = std::make_shared<Req>();
     r->key = Key{n, p_blocks, terms, leaf, dtype, dir};
     r->in = d_in;

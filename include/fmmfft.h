@@ -164,15 +164,17 @@ int fmmfft_profile_end(fmmfft_plan plan, double *stage_ms, int32_t nstages, int6
  * workspaces cached on the device under an HBM budget (evicting idle shapes,
  * least recently used first; a request that does not fit waits, at the head
  * of the queue, until completions free memory), batches same-shape requests
- * into one launch sequence replayed from a captured hipGraph (N <= 2^20; the
- * requests are gathered into and scattered from a contiguous slot), runs
- * larger shapes directly on the caller's buffers, and overlaps batches of
- * different shapes on separate streams: it launches the oldest request whose
- * shape has a free slot.  Requests carry caller-owned device buffers that
- * must stay valid until fmmfft_server_wait returns; every result equals what
+ * (up to 2^23 points per batch) into one launch sequence replayed from a
+ * captured hipGraph that reads and writes every request's own buffers in
+ * place (through a device table of their addresses; no copies), and
+ * overlaps batches of different shapes on separate streams, up to 16
+ * batches at once: it launches the oldest request whose shape has a free
+ * slot.  Requests carry caller-owned device buffers (input 16-byte aligned,
+ * output aligned to one complex point; in place allowed) that must stay
+ * valid until fmmfft_server_wait returns; every result equals what
  * fmmfft_execute returns for the same input.  bytes_in_use / peak_bytes
  * count every device allocation the server makes for its shapes (plan
- * tables, slot workspaces, gather buffers), all charged to the budget.
+ * tables, slot workspaces, address tables), all charged to the budget.
  * Latency is submit to completion on the device (a timing event after each
  * batch), not to when the scheduler notices it.
  * ------------------------------------------------------------------------- */

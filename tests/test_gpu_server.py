@@ -1,6 +1,8 @@
 """GPU tests of the request server (fmmfft_server_*, through the C ABI):
 every result of a mixed request stream equals the direct execute of the same
-input (bit for bit: batched and single launches run the same arithmetic),
+input (bit for bit: batched and single launches run the same arithmetic,
+and a batch reads and writes every request's own buffers in place through
+a device table of addresses),
 requests that cannot fit the HBM budget fail cleanly, idle shapes are
 evicted to admit new ones, and the statistics add up."""
 import ctypes
@@ -51,9 +53,9 @@ def make_requests(torch, fmm, count, seed, kmin=12, kmax=24):
 
 
 def test_mixed_stream_matches_direct_execute(fmm, orc, torch):
-    """64 requests, k in 12..24 (N above 2^20 takes the direct path), both
-    precisions, terms 12 and 20: every output equals the direct execute bit
-    for bit, and one request per (k, dtype) matches the oracle's FP64 FFT."""
+    """64 requests, k in 12..24, both precisions, terms 12 and 20: every
+    output equals the direct execute bit for bit, and one request per (k,
+    dtype) matches the oracle's FP64 FFT."""
     say("server mixed stream")
     reqs = make_requests(torch, fmm, 64, 1)
     ks = {(r[0][0].bit_length() - 1, r[0][4]) for r in reqs}
@@ -142,11 +144,12 @@ def test_idle_shapes_are_evicted(fmm, torch):
     b = shape_for(16, 20, True)
     _, wa = fmm.plan_sizes(*a)
     _, wb = fmm.plan_sizes(*b)
-    # room for one shape (tables, two slots of workspace + gather buffers) at a time
+    # room for one shape (tables, two slots of workspace and address table)
+    # at a time
     ta, _ = fmm.plan_sizes(*a)
     tb, _ = fmm.plan_sizes(*b)
-    na = ta + 2 * wa + 4 * a[0] * 8
-    nb = tb + 2 * wb + 4 * b[0] * 16
+    na = ta + 2 * (wa + 16)
+    nb = tb + 2 * (wb + 16)
     budget = max(na, nb) + min(na, nb) // 2
     outs = []
     for i in range(6):
@@ -194,3 +197,34 @@ def test_submit_waits_for_the_callers_stream(fmm, torch):
     srv.close()
     torch.cuda.synchronize()
     assert torch.equal(out, fmm.Plan(n, p, t, leaf, dt).execute(src))
+
+
+def test_large_requests_batch_in_place(fmm, torch):
+    """Requests above 2^20 points are batched too (2^21: four per batch),
+    half of them in place (input buffer = output buffer): every result
+    equals the direct out-of-place execute bit for bit, and the eight
+    requests took fewer than eight launches."""
+    say("server large batches in place")
+    n, p, t, leaf, dt = shape_for(21, 20, False)
+    xs = [torch.randn(n, dtype=torch.complex64, device="cuda") for _ in range(8)]
+    refs = []
+    plan = fmm.Plan(n, p, t, leaf, dt)
+    for x in xs:
+        refs.append(plan.execute(x))
+    bufs = [x.clone() if i % 2 == 0 else torch.empty_like(x) for i, x in enumerate(xs)]
+    torch.cuda.synchronize()
+    srv = fmm.Server(16 << 30, max_batch=64)
+    tickets = []
+    for i, x in enumerate(xs):
+        if i % 2 == 0:
+            tickets.append(srv.submit(bufs[i], bufs[i], n, p, t, leaf, dt))  # in place
+        else:
+            tickets.append(srv.submit(x, bufs[i], n, p, t, leaf, dt))
+    for tk in tickets:
+        srv.wait(tk)
+    st = srv.stats()
+    srv.close()
+    torch.cuda.synchronize()
+    for i in range(8):
+        assert torch.equal(bufs[i], refs[i]), i
+    assert st["completed"] == 8 and st["batches"] < 8
