@@ -14,14 +14,23 @@
 #include "internal.hpp"
 #include "kernels_fft.hpp"
 #include "kernels_fmm.hpp"
-#include "kernels_ops.hpp"
+#ifndef FMMFFT_TAB_UNIT
+#include "kernels_ops.hpp"  // table construction: plan creation only
+#endif
 
+// This file is compiled twice: as itself (plans, the C ABI, executes on
+// contiguous batches) and, with FMMFFT_TAB_UNIT defined, by fmmfft_tab.hip
+// for the table-addressed executes of the server (fmmfft::execute_table),
+// so that the two sets of kernel instances build in parallel.
 using namespace fmmfft;
+
+#ifndef FMMFFT_TAB_UNIT
+std::atomic<int> fmmfft::g_last_hip_error{0};
+#endif
 
 namespace {
 
-std::atomic<int> g_last_hip_error{0};
-
+#ifndef FMMFFT_TAB_UNIT
 // Workspace of fmmfft_execute_static: one array per device, held for the
 // life of the process.  Host code never uses this name as an address (on
 // the host it is only a stand-in): the device address comes from
@@ -48,6 +57,7 @@ void *static_scratch(int dev) {
     }
     return g_scratch_ptr[dev];
 }
+#endif
 
 constexpr size_t kAlign = 256;
 inline size_t align_up(size_t v) { return (v + kAlign - 1) / kAlign * kAlign; }
@@ -880,6 +890,7 @@ int run(const fmmfft_plan_s *p, int64_t count, SigAddr<const cx<T>> x, SigAddr<c
 // C ABI
 // ---------------------------------------------------------------------------
 
+#ifndef FMMFFT_TAB_UNIT
 extern "C" {
 
 int fmmfft_plan_sizes(int64_t n, int32_t p_blocks, int32_t terms, int32_t leaf, int32_t dtype,
@@ -1123,7 +1134,9 @@ int fmmfft_execute_ws(fmmfft_plan plan, int64_t count, const void *d_in, void *d
 }
 
 }  // extern "C"
+#endif  // !FMMFFT_TAB_UNIT
 
+#ifdef FMMFFT_TAB_UNIT
 int fmmfft::execute_table(fmmfft_plan plan, int64_t count, const void *const *d_in_tab, void *const *d_out_tab,
                           int32_t dir, void *d_work, size_t work_bytes, void *stream) {
     if (!plan || !d_in_tab || !d_out_tab || !d_work || count <= 0 || count > 65535) return FMMFFT_EINVAL;
@@ -1138,6 +1151,7 @@ int fmmfft::execute_table(fmmfft_plan plan, int64_t count, const void *const *d_
                              SigAddr<c64>{nullptr, 0, reinterpret_cast<c64 *const *>(d_out_tab)}, dir,
                              (char *)d_work, st);
 }
+#else
 
 extern "C" {
 
@@ -1238,3 +1252,4 @@ const char *fmmfft_strerror(int status) {
 int fmmfft_last_hip_error(void) { return g_last_hip_error.load(); }
 
 }  // extern "C"
+#endif  // FMMFFT_TAB_UNIT

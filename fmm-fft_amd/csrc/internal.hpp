@@ -3,9 +3,14 @@
 #pragma once
 #include <stdint.h>
 
+#include <atomic>
+
 #include "../../include/fmmfft.h"
 
 namespace fmmfft {
+
+// last HIP error any library call saw (fmmfft_last_hip_error)
+__attribute__((visibility("hidden"))) extern std::atomic<int> g_last_hip_error;
 
 // fmmfft_plan_create, optionally without the default workspace (the plan
 // can then only execute through fmmfft_execute_ws / fmmfft_execute_static).

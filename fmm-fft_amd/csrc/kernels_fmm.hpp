@@ -151,7 +151,8 @@ __global__ __launch_bounds__(kS2mThreads) void k_s2m(Geom g, SigAddr<const cx<T>
 // sumsec[sc] = (i/M) * sum over blocks of the partial sums (the reference's
 // MPI_Reduce + MPI_Bcast + scaling, src/fmmfft.c:452-457).  Deterministic:
 // fixed strided order and a fixed tree.
-__global__ __launch_bounds__(256) void k_sums_reduce(Geom g, const c64 *__restrict__ sums_part, int nblk,
+// (internal linkage: both translation units of the library launch it)
+static __global__ __launch_bounds__(256) void k_sums_reduce(Geom g, const c64 *__restrict__ sums_part, int nblk,
                                                      int64_t sums_stride, c64 *__restrict__ sumsec,
                                                      int64_t sumsec_stride) {
     __shared__ double rre[256], rim[256];
