@@ -310,7 +310,8 @@ def stage_counts(esize=8):
     to 2, the downward pass reads phi_l and psi_(l-1)
     and writes psi_l for l = 2..n (one parent L2L and three M2L flips per
     node, one flip at the top), except that a level formed and consumed
-    inside one launch (the paired tree kernels) is not read back; the leaf
+    inside one launch (the paired tree kernels, downward up to level 10) is
+    not read back; the leaf
     reads x and psi_n and writes the section sums; the combine and each FFT
     pass read and write N points.  The complex64 leaf runs its near field
     (4 S M 3b flops) as three FP16 split products on the FP16 matrix pipe
@@ -326,6 +327,8 @@ def stage_counts(esize=8):
     nb = N * esize
     fused = 2 if (p <= 16 and n >= 4) else 1  # M2M levels formed by the S2M kernel
     paired = p <= 16 and P_BLOCKS >= 32        # tree levels two per launch
+    # (downward pairs up to level 10, the finer levels one per launch in the
+    # sibling form)
     up_bytes, lu = 0, n - 1 - fused             # the library's launch sequence
     while paired and lu - 1 >= 2:
         up_bytes += lvl(lu + 1) + lvl(lu) + lvl(lu - 1)
@@ -334,7 +337,7 @@ def stage_counts(esize=8):
         up_bytes += lvl(l + 1) + lvl(l)
     down_bytes, l = 2 * lvl(2), 3               # top flip, then levels 3..n
     if paired:
-        last = n - 2
+        last = min(n - 2, 10)
         if (last - 2) % 2 == 1:
             down_bytes += 2 * lvl(l) + lvl(l - 1)
             l += 1
