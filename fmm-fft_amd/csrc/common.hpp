@@ -73,6 +73,20 @@ struct SigAddr {
     }
 };
 
+// Loads and stores of one complex point with the nontemporal hint (data
+// streamed once: not kept in the caches for reuse).
+template <typename T>
+__device__ inline cx<T> ld_nt(const cx<T> *p) {
+    typedef T v2 __attribute__((ext_vector_type(2)));
+    const v2 t = __builtin_nontemporal_load(reinterpret_cast<const v2 *>(p));
+    return cx<T>{t.x, t.y};
+}
+template <typename T>
+__device__ inline void st_nt(cx<T> *p, cx<T> v) {
+    typedef T v2 __attribute__((ext_vector_type(2)));
+    __builtin_nontemporal_store(v2{v.re, v.im}, reinterpret_cast<v2 *>(p));
+}
+
 // Row order of the leaf output (psec) and of the combine output.  Natural
 // (ltw < 0), or column-tiled for a two-pass local FFT of M = M1 x M2
 // (M1 = 2^lm1, M2 = 2^lm2, row mu = n1 M2 + n2): row mu at
