@@ -228,3 +228,32 @@ def test_large_requests_batch_in_place(fmm, torch):
     for i in range(8):
         assert torch.equal(bufs[i], refs[i]), i
     assert st["completed"] == 8 and st["batches"] < 8
+
+
+@pytest.mark.parametrize("n,p,terms,leaf", [(1 << 22, 2, 16, 32),   # M = 2^21: three-pass local FFT
+                                            (1 << 14, 2, 12, 16),   # P = 2
+                                            (1 << 15, 4, 20, 32)])  # P = 4
+def test_other_layouts_through_the_server(fmm, torch, n, p, terms, leaf):
+    """Shapes the serving stream does not draw (the three-pass local FFT,
+    P = 2 and 4), both precisions, three requests each in one batch: every
+    result equals the direct execute bit for bit."""
+    say(f"server layouts n={n} p={p}")
+    srv = fmm.Server(16 << 30, max_batch=8)
+    cases = []
+    for dt in (0, 1):
+        dtype = torch.complex128 if dt == 1 else torch.complex64
+        g = torch.Generator(device="cuda").manual_seed(n + dt)
+        xs = [torch.randn(n, dtype=dtype, device="cuda", generator=g) for _ in range(3)]
+        outs = [torch.empty_like(x) for x in xs]
+        cases.append((dt, xs, outs))
+    torch.cuda.synchronize()
+    tickets = [srv.submit(x, o, n, p, terms, leaf, dt) for dt, xs, outs in cases for x, o in zip(xs, outs)]
+    for t in tickets:
+        srv.wait(t)
+    srv.close()
+    torch.cuda.synchronize()
+    for dt, xs, outs in cases:
+        plan = fmm.Plan(n, p, terms, leaf, dt)
+        for x, o in zip(xs, outs):
+            assert torch.equal(o, plan.execute(x)), (n, p, dt)
+        plan.close()
