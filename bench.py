@@ -428,11 +428,13 @@ def main():
         gflops64 = 5 * N * math.log2(N) / (ms64 * 1e-3) / 1e9
         d = out32.to(torch.complex128) - out64
         rel = float(torch.linalg.vector_norm(d) / torch.linalg.vector_norm(out64))
+        maxerr = float(d.abs().max() / out64.abs().max())
         extra.update({
             "fp64_plain_form": {"value": round(gflops64, 1), "unit": "GFLOP/s", "ms_per_step": round(ms64, 4),
                                 "stage_ms": {k: round(v / steps64, 4) for k, v in stages64[0].items()}},
             "speedup_fp32_vs_fp64": round(gflops32 / gflops64, 3),
             "rel_l2_fp32_vs_fp64": rel,
+            "max_err_fp32_vs_fp64": maxerr,  # max |X32 - X64| / max |X64|
         })
         del plan64, xin64, out64, d
         # the split-product M2L form (FMMFFT_C64_SPLIT) on the same input

@@ -875,6 +875,15 @@ int run(const fmmfft_plan_s *p, int64_t count, SigAddr<const cx<T>> x, SigAddr<c
             launch_fft<T, TAB>(f.M3, g, b3, nz, st, psec_in, out, twC, tlo, thi);
         }
     }
+    // complex64 with 12 terms or more: the P block DC bins from the FP64
+    // section sums.  Those bins are then the exact DFT's, where the FMM-FFT
+    // carries its truncation error (below 1.1e-11 of the result from 12
+    // terms on, far under the complex64 rounding; with fewer terms the
+    // transform keeps the FMM-FFT's own values)
+    if constexpr (sizeof(T) == 4)
+        if (g.terms >= 12)
+            hipLaunchKernelGGL((k_dc_exact<T, TAB>), dim3(1, 1, nz), dim3(256), 0, st, g, (const c64 *)sums_part,
+                           (int)sh.s2m_nblk, wstride_c64, (const c64 *)sumsec, wstride_c64, wP, out, conj);
     mark(7);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) {

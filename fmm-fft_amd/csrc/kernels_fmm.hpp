@@ -618,6 +618,58 @@ __global__ __launch_bounds__(256) void k_s2m_mma(Geom g, SigAddr<const cx<T>> xa
     }
 }
 
+// Exact DC bins of the P blocks (complex64 output): X[M k2] = sum_n x[n]
+// e^{-2 pi i n k2 / P} = sum_sc S_sc w_P^{sc k2}, the P-point DFT of the
+// section sums S_sc = sum_mu x[mu P + sc], which the S2M kernel accumulates
+// in FP64.  In the FMM-FFT those P outputs are the sum over a whole block of
+// FP32 cot sums, whose small coherent rounding bias adds up M times (at
+// M = 2^20 their rms error was 360 times the other bins'), so they are
+// rewritten from the FP64 sums after the last FFT pass (plans of 12 terms
+// or more, where the FMM's own truncation error at those bins is far below
+// the FP32 rounding).  One workgroup per
+// signal: S_0 by the fixed-order reduction of k_sums_reduce, S_sc (sc >= 1)
+// from sumsec = (i/M) S_sc; the inverse transform conjugates as the last
+// FFT pass does (its sums are of the conjugated input).
+template <typename T, bool TAB = false>
+__global__ __launch_bounds__(256) void k_dc_exact(Geom g, const c64 *__restrict__ sums_part, int nblk,
+                                                  int64_t sums_stride, const c64 *__restrict__ sumsec,
+                                                  int64_t sumsec_stride, const c64 *__restrict__ wP,
+                                                  SigAddr<cx<T>> out, int conj) {
+    __shared__ double rre[256], rim[256];
+    __shared__ c64 S[64];
+    const int sig = blockIdx.z, P = g.P;
+    sums_part += (int64_t)sig * sums_stride;
+    sumsec += (int64_t)sig * sumsec_stride;
+    double re = 0.0, im = 0.0;
+    for (int i = threadIdx.x; i < nblk; i += blockDim.x) {
+        const c64 v = sums_part[(int64_t)i * P];
+        re += v.re;
+        im += v.im;
+    }
+    rre[threadIdx.x] = re;
+    rim[threadIdx.x] = im;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) {
+            rre[threadIdx.x] += rre[threadIdx.x + h];
+            rim[threadIdx.x] += rim[threadIdx.x + h];
+        }
+        __syncthreads();
+    }
+    if ((int)threadIdx.x < P) {
+        const int sc = threadIdx.x;
+        const double m = (double)g.M;
+        S[sc] = sc == 0 ? c64{rre[0], rim[0]} : c64{m * sumsec[sc].im, -m * sumsec[sc].re};
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < P) {
+        const int k2 = threadIdx.x;
+        c64 acc{0.0, 0.0};
+        for (int sc = 0; sc < P; ++sc) acc = cadd(acc, cmul(S[sc], wP[(sc * k2) & (P - 1)]));
+        out.template at<TAB>(sig)[(int64_t)k2 * g.M] = cx<T>{(T)acc.re, (T)(conj ? -acc.im : acc.im)};
+    }
+}
+
 // Waves per SIMD the register allocation of a leaf instance is held to:
 // three for the FP32 leaf-64 instances, which fit 168 registers without
 // spilling (left alone they take ~180 and run two workgroups per CU), four

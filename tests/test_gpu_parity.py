@@ -335,6 +335,31 @@ def test_fp32_precision_form_full_size(fmm, orc, torch):
         assert err < tol, (dtype, err)
 
 
+@pytest.mark.parametrize("direction", [1, -1])
+def test_fp32_block_dc_bins_exact(fmm, orc, torch, direction):
+    """complex64, 16 terms, N=2^22 (M=2^16), P=64, N(0,1) input: the P block
+    DC bins X[M k2] (the P-point DFT of the section sums, written from the
+    FP64 sums) equal the oracle's FP64 FFT to the FP32 rounding of the value,
+    and the largest error over all bins stays within 1e-6 of the largest
+    output (the FMM's FP32 sums put 1.7e-6 there before the DC bins were
+    written exactly), forward and inverse."""
+    say(f"fp32 dc bins dir={direction}")
+    n, p = 1 << 22, 64
+    m = n // p
+    x = orc.normal(n, 21).astype(np.complex64).astype(np.complex128)
+    plan = fmm.Plan(n, p, 16, 64, fmm.C64)
+    xd = torch.from_numpy(x).to(torch.complex64).cuda()
+    X = plan.execute(xd, direction=direction).cpu().numpy().astype(np.complex128)
+    plan.close()
+    Y = orc.fft64(x, -direction)
+    dc = np.arange(p) * m
+    assert np.all(np.abs(X[dc] - Y[dc]) <= 2.0 ** -23 * np.abs(Y[dc]) + 1e-9 * np.abs(Y).max()), \
+        np.max(np.abs(X[dc] - Y[dc]) / np.abs(Y[dc]))
+    worst = np.max(np.abs(X - Y)) / np.abs(Y).max()
+    say(f"  rel-L2 {rel(X, Y):.3e} max-abs/max {worst:.3e}")
+    assert worst < 1e-6 and rel(X, Y) < 1e-6
+
+
 def test_fp32_batched_form(fmm, orc, torch):
     """256 signals of N=2^20 complex64, P=16, p=12, leaf 32; per-signal scale
     10^U(-3,3); rel-L2 <= 1e-6 per signal: all 256 against numpy's FP64 FFT
