@@ -1090,12 +1090,50 @@ void k_leaf_h16(Geom g, SigAddr<const c32> xa, const float *__restrict__ psi,
 
     float xr[UPER][8];  // 4 rows x (re, im) per unit
     float lr[LPER];
+    // byte offsets of this lane's loads from a tile's first window row and
+    // first expansion box (fixed for the kernel)
+    uint32_t bx_off[UPER], bl_off[LPER];
+#pragma unroll
+    for (int u = 0; u < UPER; ++u) {
+        const int e = (NU % NTH == 0) ? threadIdx.x + NTH * u : min((int)threadIdx.x + NTH * u, NU - 1);
+        const int rq = e / G, ge = e % G;
+        bx_off[u] = (uint32_t)(((4 * rq) << lp) + ge) * (uint32_t)sizeof(c32);
+    }
+#pragma unroll
+    for (int u = 0; u < LPER; ++u) {
+        const int e = (LN % NTH == 0) ? threadIdx.x + NTH * u : min((int)threadIdx.x + NTH * u, LN - 1);
+        const int bx9 = e / NL, rem = e % NL;
+        const int kk = rem / (2 * G), rr = rem % (2 * G);
+        const int sce = sc0 + (rr >> 1);
+        bl_off[u] = (uint32_t)((bx9 * terms + min(kk, terms - 1)) * rowsS + 2 * sce + (rr & 1)) * (uint32_t)sizeof(float);
+    }
     auto prefetch = [&](int tile) __attribute__((always_inline)) {
         const int sig = tile / ntiles, lid = tile - sig * ntiles;
         const c32 *xs = xa.template at<TAB>(sig);
         const float *ps = psi + (int64_t)sig * tree_stride;
         const int i0 = (lid / ngroups) * BB;
         const int mu_w0 = (i0 - 2) * b;
+        // tiles whose window and expansions do not wrap around the signal
+        // (all but the first and last box chunk): one per-tile base and the
+        // fixed lane offsets, so an address is one add (-8 us at N = 2^26);
+        // the other two take the periodic index row by row
+        if (i0 >= 2 && i0 + BB + 1 <= g.s) {
+            const c32 *xb = xs + ((int64_t)mu_w0 << lp) + sc0;
+            const char *pb = reinterpret_cast<const char *>(ps + (int64_t)(i0 - 1) * terms * rowsS);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const char *xk = reinterpret_cast<const char *>(xb + ((int64_t)k << lp));
+#pragma unroll
+                for (int u = 0; u < UPER; ++u) {
+                    const c32 v = *reinterpret_cast<const c32 *>(xk + bx_off[u]);
+                    xr[u][2 * k] = v.re;
+                    xr[u][2 * k + 1] = v.im;
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < LPER; ++u) lr[u] = *reinterpret_cast<const float *>(pb + bl_off[u]);
+            return;
+        }
 #pragma unroll
         for (int u = 0; u < UPER; ++u) {
             const int e = (NU % NTH == 0) ? threadIdx.x + NTH * u : min((int)threadIdx.x + NTH * u, NU - 1);
