@@ -357,6 +357,27 @@ __device__ inline double bsel(int p, double a, double b) {
 __device__ inline float swap_pair(float v) {
     return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xB1, 0xF, 0xF, true));
 }
+// value of the lane CTRL rotates in (a DPP row rotation within 16 lanes)
+template <int CTRL>
+__device__ inline float dpp_f(float v) {
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+// op (commutative) over the lanes of a wave of equal lane % G (G = 2, 4, 8
+// or 16), every lane receiving its class's result: DPP row rotations inside
+// each 16-lane row, then the row-pair and half-wave swaps of gfx950 (VALU
+// only, no LDS round trip; an LDS-permute chain took one dependent LDS
+// latency per step)
+template <int G, typename Op>
+__device__ inline float lanes_reduce_mod(float v, Op op) {
+    static_assert(G == 2 || G == 4 || G == 8 || G == 16, "lane classes of 2, 4, 8 or 16");
+    if constexpr (G == 2) v = op(v, dpp_f<0x122>(v));  // row_ror:2
+    if constexpr (G <= 4) v = op(v, dpp_f<0x124>(v));  // row_ror:4
+    if constexpr (G <= 8) v = op(v, dpp_f<0x128>(v));  // row_ror:8
+    const auto r16 = __builtin_amdgcn_permlane16_swap(__float_as_uint(v), __float_as_uint(v), false, false);
+    v = op(__uint_as_float(r16[0]), __uint_as_float(r16[1]));
+    const auto r32 = __builtin_amdgcn_permlane32_swap(__float_as_uint(v), __float_as_uint(v), false, false);
+    return op(__uint_as_float(r32[0]), __uint_as_float(r32[1]));
+}
 __device__ inline double swap_pair(double v) {
     const unsigned long long u = (unsigned long long)__double_as_longlong(v);
     const unsigned lo = (unsigned)__builtin_amdgcn_mov_dpp((int)(unsigned)u, 0xB1, 0xF, 0xF, true);
@@ -1176,8 +1197,7 @@ void k_leaf_h16(Geom g, SigAddr<const c32> xa, const float *__restrict__ psi,
             }
         }
         // lanes of equal section: lane % G
-#pragma unroll
-        for (int o = G; o < 64; o <<= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
+        m = lanes_reduce_mod<G>(m, [](float a, float b2) { return fmaxf(a, b2); });
         if (lane < G) smax[wave * G + lane] = m;
 #pragma unroll
         for (int u = 0; u < LPER; ++u) {
@@ -1289,8 +1309,7 @@ void k_leaf_h16(Geom g, SigAddr<const c32> xa, const float *__restrict__ psi,
 #pragma unroll
                     for (int j = 0; j < KT; ++j)
                         corr += eh[4 * j + kr] * lm[(4 * j + kr) * 2 * G] - e0[4 * j + kr] * lb[(4 * j + kr) * 2 * G];
-                    corr += __shfl_xor(corr, 16, 64);
-                    corr += __shfl_xor(corr, 32, 64);
+                    corr = lanes_reduce_mod<16>(corr, [](float a, float b2) { return a + b2; });
                     if (kr == 0) acc[0][0] += (2 * sc == P) ? 0.5f * corr : corr;
                 }
                 // write: pair lanes (c = 0, 1) swap halves (as k_leaf_mma)
