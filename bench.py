@@ -283,12 +283,14 @@ def measured_traffic(stage):
     return None
 
 
-def leaf_compute_ms():
+def leaf_compute_ms(fp16_rate=1.0):
     """Matrix-pipe time of the complex64 leaf at peak: near field as three
     FP16 split products (Kh Xh + Kh Xl + Kl Xh) of 4 S M 3b flops each on
-    the FP16 pipe, L2T (4 S M p flops) on the FP32 pipe."""
+    the FP16 pipe, L2T (4 S M p flops) on the FP32 pipe.  fp16_rate scales
+    the FP16 peak: the kernel issues v_mfma_f32_16x16x16_f16, which gfx950
+    runs at half its FP16 peak (the peak is reached by the K = 32 forms)."""
     M, S = N // P_BLOCKS, P_BLOCKS - 1
-    return (3 * 4.0 * S * M * 3 * LEAF / (PEAK_FP16_TFLOPS * 1e12) +
+    return (3 * 4.0 * S * M * 3 * LEAF / (fp16_rate * PEAK_FP16_TFLOPS * 1e12) +
             4.0 * S * M * TERMS / (PEAK_FP32_TFLOPS * 1e12)) * 1e3
 
 
@@ -465,7 +467,8 @@ def main():
         roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                 "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic, "kernel": "k_leaf_h16<4,4,4>",
                 "algorithmic": f"{by:.4g} bytes per launch (x and psi_n read, psec written); matrix-pipe time "
-                               f"at peak {leaf_compute_ms():.4f} ms"}
+                               f"at peak {leaf_compute_ms():.4f} ms, {leaf_compute_ms(0.5):.4f} ms at the rate "
+                               f"of the K=16 FP16 MFMA it issues"}
     else:
         by = 2.0 * N * 8  # one read + one write of the complex64 signal
         achieved = by / (per_stage[dom] * 1e-3) / 1e9
