@@ -21,6 +21,7 @@ CASES = [
     (1 << 14, 16, 32, 16),
     (1 << 13, 4, 17, 32),
     (1 << 15, 64, 8, 32),
+    (1 << 15, 2, 12, 32),    # complex64: FP16 leaf with two sections per workgroup
     (1 << 16, 32, 24, 32),
     (1 << 14, 2, 31, 64),
     (1 << 16, 8, 16, 64),
