@@ -640,9 +640,6 @@ void s2m_tree_pt(const Geom &g, const Shapes &sh, unsigned nz, hipStream_t st, S
     else
         hipLaunchKernelGGL((k_s2m<T, PT, TAB>), dim3(sh.s2m_nblk, 1, nz), dim3(kS2mThreads), sh.s2m_lds, st, g, x,
                            phi, wstride_T, sums_part, wstride_c64, tb.initch, conj);
-    hipLaunchKernelGGL(k_sums_reduce, dim3(g.P, 1, nz), dim3(256), 0, st, g, sums_part, sh.s2m_nblk, wstride_c64,
-                       sumsec, wstride_c64);
-    mark(1);
     // tree passes on the matrix cores: terms padded to 16 or 32, one wave per
     // (node, block of 16 rows), four waves per workgroup
     constexpr int TP = PT <= 16 ? 16 : 32;
@@ -650,6 +647,16 @@ void s2m_tree_pt(const Geom &g, const Shapes &sh, unsigned nz, hipStream_t st, S
     // nodes of at most two row blocks (P <= 16): packed form, several
     // nodes per wave
     const bool pack = nrb <= 2;
+    // the matrix-core S2M has already formed levels n-1 (and n-2)
+    int lu = g.n - 1 - (sh.s2m_vec ? sh.s2m_levels : 0);
+    // complex64, 16 terms, nodes of more than two row blocks: upward levels
+    // two per launch, the first of which also reduces the section sums
+    const bool up2 = sizeof(T) == 4 && TP == 16 && !pack && lu - 1 >= 2;
+    const SumsJob sj{sums_part, (int)sh.s2m_nblk, wstride_c64, sumsec, wstride_c64};
+    // downward levels: the split-product M2L instances for FMMFFT_C64_SPLIT
+    const bool split = tb.opfrag_lo != nullptr;
+    if (!up2) hipLaunchKernelGGL(k_sums_reduce, dim3(g.P, 1, nz), dim3(256), 0, st, g, sj);
+    mark(1);
     auto grid = [&](int l, int mode) {
         const int64_t nn = int64_t(1) << l;
         int64_t waves;
@@ -666,8 +673,6 @@ void s2m_tree_pt(const Geom &g, const Shapes &sh, unsigned nz, hipStream_t st, S
         hipLaunchKernelGGL(pack ? pkern : kern, grid(l, mode), dim3(256), 0, st, g, l, (const T *)phi, phi,
                            (const T *)psi, psi, wstride_T, tb.opfrag, tb.opfrag_lo);
     };
-    // downward levels: the split-product M2L instances for FMMFFT_C64_SPLIT
-    const bool split = tb.opfrag_lo != nullptr;
     // complex64 16-term trees whose nodes are whole 64-row groups (P >= 32):
     // the finest downward levels in the sibling form
     const bool sib4 = sizeof(T) == 4 && TP == 16 && !pack && tree_rows(g) % 64 == 0;
@@ -680,19 +685,34 @@ void s2m_tree_pt(const Geom &g, const Shapes &sh, unsigned nz, hipStream_t st, S
             else level(k_tree_mma<T, TP, 1, false>, k_tree_mma<T, TP, 1, true>, l, 1);
         }
     };
-    // the matrix-core S2M has already formed levels n-1 (and n-2)
-    int lu = g.n - 1 - (sh.s2m_vec ? sh.s2m_levels : 0);
+    // levels (l, l - 1) two per launch; the launch that ends at level 2 also
+    // applies the top flip (not in the split-M2L form, whose top flip takes
+    // the split operator)
+    bool top_done = false;
     if constexpr (sizeof(T) == 4 && TP == 16) {
-        // complex64, 16 terms, nodes of more than two row blocks: levels
-        // (l, l - 1) two per launch
-        if (!pack)
-            for (; lu - 1 >= 2; lu -= 2)
-                hipLaunchKernelGGL((k_tree_up2<T, TP>), dim3((unsigned)(((int64_t(1) << (lu - 1)) * nrb + 3) / 4), 1, nz),
-                                   dim3(256), 0, st, g, lu, phi, wstride_T, tb.opfrag);
+        if (up2)
+            for (bool first = true; lu - 1 >= 2; lu -= 2, first = false) {
+                const unsigned nt = (unsigned)(((int64_t(1) << (lu - 1)) * nrb + 3) / 4);
+                const bool topf = lu - 1 == 2 && !split;
+                const dim3 gd(nt + (first ? (unsigned)g.P : 0u), 1, nz);
+                if (first && topf)
+                    hipLaunchKernelGGL((k_tree_up2<T, TP, true, true>), gd, dim3(256), 0, st, g, lu, phi, wstride_T,
+                                       tb.opfrag, psi, sj);
+                else if (first)
+                    hipLaunchKernelGGL((k_tree_up2<T, TP, true, false>), gd, dim3(256), 0, st, g, lu, phi, wstride_T,
+                                       tb.opfrag, psi, sj);
+                else if (topf)
+                    hipLaunchKernelGGL((k_tree_up2<T, TP, false, true>), gd, dim3(256), 0, st, g, lu, phi, wstride_T,
+                                       tb.opfrag, psi, sj);
+                else
+                    hipLaunchKernelGGL((k_tree_up2<T, TP, false, false>), gd, dim3(256), 0, st, g, lu, phi,
+                                       wstride_T, tb.opfrag, psi, sj);
+                top_done = topf;
+            }
     }
     for (; lu >= 2; --lu) level(k_tree_mma<T, TP, 0, false>, k_tree_mma<T, TP, 0, true>, lu, 0);
     mark(2);
-    down(2, 2);
+    if (!top_done) down(2, 2);
     int l = 3;
     if constexpr (sizeof(T) == 4 && TP == 16) {
         // complex64, 16 terms, nodes of more than two row blocks: levels

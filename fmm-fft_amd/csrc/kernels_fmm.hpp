@@ -151,13 +151,20 @@ __global__ __launch_bounds__(kS2mThreads) void k_s2m(Geom g, SigAddr<const cx<T>
 // sumsec[sc] = (i/M) * sum over blocks of the partial sums (the reference's
 // MPI_Reduce + MPI_Bcast + scaling, src/fmmfft.c:452-457).  Deterministic:
 // fixed strided order and a fixed tree.
-// (internal linkage: both translation units of the library launch it)
-static __global__ __launch_bounds__(256) void k_sums_reduce(Geom g, const c64 *__restrict__ sums_part, int nblk,
-                                                     int64_t sums_stride, c64 *__restrict__ sumsec,
-                                                     int64_t sumsec_stride) {
+struct SumsJob {
+    const c64 *part;  // [signal][nblk][P] per-workgroup partial sums
+    int nblk;
+    int64_t part_stride;
+    c64 *sec;  // [signal][P]
+    int64_t sec_stride;
+};
+// One 256-thread workgroup's share (section sc of signal sig): a workgroup
+// of k_sums_reduce, or one of the first workgroups of the first paired
+// upward launch (which saves the reduction its own launch).
+__device__ inline void sums_reduce_block(const Geom &g, const SumsJob &j, int sc, int sig) {
     __shared__ double rre[256], rim[256];
-    const int sc = blockIdx.x, sig = blockIdx.z;
-    sums_part += (int64_t)sig * sums_stride;
+    const c64 *__restrict__ sums_part = j.part + (int64_t)sig * j.part_stride;
+    const int nblk = j.nblk;
     double re = 0.0, im = 0.0;
     for (int i = threadIdx.x; i < nblk; i += blockDim.x) {
         c64 v = sums_part[(int64_t)i * g.P + sc];
@@ -176,9 +183,12 @@ static __global__ __launch_bounds__(256) void k_sums_reduce(Geom g, const c64 *_
     }
     if (threadIdx.x == 0) {
         const double inv = 1.0 / (double)g.M;
-        sumsec[(int64_t)sig * sumsec_stride + sc] =
-            sc == 0 ? c64{0.0, 0.0} : c64{-rim[0] * inv, rre[0] * inv};
+        j.sec[(int64_t)sig * j.sec_stride + sc] = sc == 0 ? c64{0.0, 0.0} : c64{-rim[0] * inv, rre[0] * inv};
     }
+}
+// (internal linkage: both translation units of the library launch it)
+static __global__ __launch_bounds__(256) void k_sums_reduce(Geom g, SumsJob j) {
+    sums_reduce_block(g, j, blockIdx.x, blockIdx.z);
 }
 
 // ---------------------------------------------------------------------------
@@ -1636,18 +1646,31 @@ __global__ __launch_bounds__(256) void k_tree_mma(Geom g, int l, const T *__rest
 // phi_l[2q + h] from their children phi_{l+1}[4q + 2h + {0, 1}] and then
 // phi_{l-1}[q] from the two accumulators (as B operands in the D layout's K
 // order), storing all three (src/fmmfft.c:476-515).
-template <typename T, int TP>
+// SUMS: the first P workgroups reduce the section sums instead (no tree
+// work).  TOPF (l = 3): the top flip as well, psi_2[(q + 2) mod 4] = topflp
+// phi_2[q] (src/fmmfft.c:620-639) from the phi_2 accumulators, which saves
+// the top flip its launch.
+template <typename T, int TP, bool SUMS = false, bool TOPF = false>
 __global__ __launch_bounds__(256) void k_tree_up2(Geom g, int l, T *__restrict__ phi, int64_t tree_stride,
-                                                  const T *__restrict__ opfrag) {
+                                                  const T *__restrict__ opfrag, T *__restrict__ psi, SumsJob sj) {
     using V4 = typename Mfma<T>::v4;
     static_assert(TP == 16, "one 16-term block");
     constexpr int KS = TP / 4, FR = TP * TP;
+    int bid = blockIdx.x, nb = gridDim.x;
+    if constexpr (SUMS) {
+        if (bid < g.P) {
+            sums_reduce_block(g, sj, bid, blockIdx.z);
+            return;
+        }
+        bid -= g.P;
+        nb -= g.P;
+    }
     const int terms = g.terms, rows = tree_rows(g);
     const int node_len = rows * terms;
     const int nq = 1 << (l - 1), nrb = (rows + 15) >> 4;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     const int col = lane & 15, kr = lane >> 4;
-    const int wid = 4 * (int)xcd_remap(blockIdx.x, gridDim.x) + wave;
+    const int wid = 4 * (int)xcd_remap(bid, nb) + wave;
     const int q = wid / nrb, rb = wid - q * nrb;
     if (q >= nq) return;
     const int64_t ts = (int64_t)blockIdx.z * tree_stride;
@@ -1658,7 +1681,8 @@ __global__ __launch_bounds__(256) void k_tree_up2(Geom g, int l, T *__restrict__
     const T *o1 = opfrag + (size_t)(l - 2) * NOPS * FR, *o2 = opfrag + (size_t)(l - 3) * NOPS * FR;
     const T *m1[2] = {o1 + OP_SHFTFN * FR, o1 + OP_SHFTFP * FR};
     const T *m2[2] = {o2 + OP_SHFTFN * FR, o2 + OP_SHFTFP * FR};
-    T a[2][KS], a2[2][KS], bv[4][KS];
+    T a[2][KS], a2[2][KS], a3[TOPF ? KS : 1], bv[4][KS];
+    const T *ot = opfrag + (size_t)(g.n - 2) * NOPS * FR;  // topflp
 #pragma unroll
     for (int c2 = 0; c2 < 2; ++c2)
 #pragma unroll
@@ -1666,6 +1690,8 @@ __global__ __launch_bounds__(256) void k_tree_up2(Geom g, int l, T *__restrict__
             a[c2][s] = m1[c2][s * 64 + lane];
             const int j = 16 * (s / 4) + Mfma<T>::row(kr, s % 4);  // D-as-B K order
             a2[c2][s] = m2[c2][(j / 4) * 64 + col + 16 * (j % 4)];
+            if constexpr (TOPF)
+                if (c2 == 0) a3[s] = ot[(j / 4) * 64 + col + 16 * (j % 4)];
         }
 #pragma unroll
     for (int k = 0; k < 4; ++k)
@@ -1686,6 +1712,10 @@ __global__ __launch_bounds__(256) void k_tree_up2(Geom g, int l, T *__restrict__
         g1 = Mfma<T>::mma(a2[1][s], up[1][s % 4], g1);
     }
     const V4 gv = g0 + g1;
+    V4 pt = V4{0, 0, 0, 0};
+    if constexpr (TOPF)
+#pragma unroll
+        for (int s = 0; s < KS; ++s) pt = Mfma<T>::mma(a3[s], gv[s % 4], pt);
     const int row = 16 * rb + col;
     if (row < rows) {
         auto store = [&](T *dst, const V4 &v) __attribute__((always_inline)) {
@@ -1698,6 +1728,7 @@ __global__ __launch_bounds__(256) void k_tree_up2(Geom g, int l, T *__restrict__
         store(pa, up[0]);
         store(pa + node_len, up[1]);
         store(gp, gv);
+        if constexpr (TOPF) store(psi + ts + level_off(g, 2) + (int64_t)((q + 2) & 3) * node_len, pt);
     }
 }
 
