@@ -321,9 +321,15 @@ WsLayout ws_layout(const Geom &g, size_t esize, const Shapes &sh) {
     WsLayout w{};
     size_t o = 0;
     w.sums_part = o; o = align_up(o + (size_t)sh.s2m_nblk * g.P * 16);
-    w.sumsec = o; o = align_up(o + (size_t)g.P * 16);
+    w.sumsec = o; o = align_up(o + (size_t)(g.P + 1) * 16);
     const size_t tl = (size_t)tree_len(g) * esize;
     w.phi = o; o = align_up(o + tl);
+    // the leaf reads the finest local expansions beside the rows of x (both
+    // at the same offsets inside 2P-real rows): starting them at an odd
+    // multiple of 256 bytes puts their requests on the other half of the
+    // 256-byte-interleaved memory channels (N = 2^26, P = 64: leaf 0.437 ->
+    // 0.427 ms against a multiple of 512)
+    if ((o + (size_t)level_off(g, g.n) * esize) % 512 < 256) o += 256;
     w.psi = o; o = align_up(o + tl);
     w.psec = o; o = align_up(o + (size_t)g.N * 2 * esize);
     w.stride = o;
@@ -902,8 +908,8 @@ int run(const fmmfft_plan_s *p, int64_t count, SigAddr<const cx<T>> x, SigAddr<c
     // transform keeps the FMM-FFT's own values)
     if constexpr (sizeof(T) == 4)
         if (g.terms >= 12)
-            hipLaunchKernelGGL((k_dc_exact<T, TAB>), dim3(1, 1, nz), dim3(256), 0, st, g, (const c64 *)sums_part,
-                           (int)sh.s2m_nblk, wstride_c64, (const c64 *)sumsec, wstride_c64, wP, out, conj);
+            hipLaunchKernelGGL((k_dc_exact<T, TAB>), dim3(1, 1, nz), dim3(64), 0, st, g, (const c64 *)sumsec,
+                               wstride_c64, wP, out, conj);
     mark(7);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) {

@@ -155,7 +155,7 @@ struct SumsJob {
     const c64 *part;  // [signal][nblk][P] per-workgroup partial sums
     int nblk;
     int64_t part_stride;
-    c64 *sec;  // [signal][P]
+    c64 *sec;  // [signal][P + 1]: (i/M) S_sc (0 for sc = 0), then S_0
     int64_t sec_stride;
 };
 // One 256-thread workgroup's share (section sc of signal sig): a workgroup
@@ -183,7 +183,9 @@ __device__ inline void sums_reduce_block(const Geom &g, const SumsJob &j, int sc
     }
     if (threadIdx.x == 0) {
         const double inv = 1.0 / (double)g.M;
-        j.sec[(int64_t)sig * j.sec_stride + sc] = sc == 0 ? c64{0.0, 0.0} : c64{-rim[0] * inv, rre[0] * inv};
+        c64 *sec = j.sec + (int64_t)sig * j.sec_stride;
+        sec[sc] = sc == 0 ? c64{0.0, 0.0} : c64{-rim[0] * inv, rre[0] * inv};
+        if (sc == 0) sec[g.P] = c64{rre[0], rim[0]};  // S_0 itself, for the exact DC bins
     }
 }
 // (internal linkage: both translation units of the library launch it)
@@ -657,55 +659,28 @@ __global__ __launch_bounds__(256) void k_s2m_mma(Geom g, SigAddr<const cx<T>> xa
 // M = 2^20 their rms error was 360 times the other bins'), so they are
 // rewritten from the FP64 sums after the last FFT pass (plans of 12 terms
 // or more, where the FMM's own truncation error at those bins is far below
-// the FP32 rounding).  One workgroup per
-// signal: S_0 by the fixed-order reduction of k_sums_reduce, S_sc (sc >= 1)
-// from sumsec = (i/M) S_sc; the inverse transform conjugates as the last
-// FFT pass does (its sums are of the conjugated input).
+// the FP32 rounding).  One workgroup per signal: S_0 as the section-sum
+// reduction left it (sumsec[P]), S_sc (sc >= 1) from sumsec = (i/M) S_sc;
+// the inverse transform conjugates as the last FFT pass does (its sums are
+// of the conjugated input).
 template <typename T, bool TAB = false>
-__global__ __launch_bounds__(256) void k_dc_exact(Geom g, const c64 *__restrict__ sums_part, int nblk,
-                                                  int64_t sums_stride, const c64 *__restrict__ sumsec,
-                                                  int64_t sumsec_stride, const c64 *__restrict__ wP,
-                                                  SigAddr<cx<T>> out, int conj) {
-    __shared__ double rre[256], rim[256];
+__global__ __launch_bounds__(64) void k_dc_exact(Geom g, const c64 *__restrict__ sumsec, int64_t sumsec_stride,
+                                                 const c64 *__restrict__ wP, SigAddr<cx<T>> out, int conj) {
     __shared__ c64 S[64], W[64];
-    const int sig = blockIdx.z, P = g.P;
-    sums_part += (int64_t)sig * sums_stride;
+    const int sig = blockIdx.z, P = g.P, t = threadIdx.x;
     sumsec += (int64_t)sig * sumsec_stride;
-    // the P twiddles and section sums are read once, before the reduction,
-    // and the DFT below reads them from LDS (7.7 -> 6.9 us per execute)
-    c64 ss{0.0, 0.0};
-    if ((int)threadIdx.x < P) {
-        W[threadIdx.x] = wP[threadIdx.x];
-        ss = sumsec[threadIdx.x];
-    }
-    double re = 0.0, im = 0.0;
-    for (int i = threadIdx.x; i < nblk; i += blockDim.x) {
-        const c64 v = sums_part[(int64_t)i * P];
-        re += v.re;
-        im += v.im;
-    }
-    rre[threadIdx.x] = re;
-    rim[threadIdx.x] = im;
-    __syncthreads();
-    for (int h = 128; h > 0; h >>= 1) {
-        if ((int)threadIdx.x < h) {
-            rre[threadIdx.x] += rre[threadIdx.x + h];
-            rim[threadIdx.x] += rim[threadIdx.x + h];
-        }
-        __syncthreads();
-    }
-    if ((int)threadIdx.x < P) {
-        const int sc = threadIdx.x;
+    if (t < P) {
+        const c64 ss = sumsec[t];
         const double m = (double)g.M;
-        S[sc] = sc == 0 ? c64{rre[0], rim[0]} : c64{m * ss.im, -m * ss.re};
+        W[t] = wP[t];
+        S[t] = t == 0 ? sumsec[P] : c64{m * ss.im, -m * ss.re};
     }
     __syncthreads();
-    if ((int)threadIdx.x < P) {
-        const int k2 = threadIdx.x;
+    if (t < P) {
         c64 acc{0.0, 0.0};
 #pragma unroll 8
-        for (int sc = 0; sc < P; ++sc) acc = cadd(acc, cmul(S[sc], W[(sc * k2) & (P - 1)]));
-        out.template at<TAB>(sig)[(int64_t)k2 * g.M] = cx<T>{(T)acc.re, (T)(conj ? -acc.im : acc.im)};
+        for (int sc = 0; sc < P; ++sc) acc = cadd(acc, cmul(S[sc], W[(sc * t) & (P - 1)]));
+        out.template at<TAB>(sig)[(int64_t)t * g.M] = cx<T>{(T)acc.re, (T)(conj ? -acc.im : acc.im)};
     }
 }
 
