@@ -96,10 +96,12 @@ def test_stages_match_oracle_fp64(fmm, orc, torch, n, p, terms, leaf):
 
 # complex64 plans with 2P >= 32 run the matrix-core S2M, which also forms
 # tree level n-1 (sibling pairs: depth 3, or more than 16 terms) or levels
-# n-1 and n-2 (sibling quads), and (P >= 32, 16 terms) the downward levels
+# n-1 and n-2 (sibling quads), and (P >= 32, 16 terms) the upward levels run
+# two per launch (the first launch also reducing the section sums, the one
+# ending at level 2 also applying the top flip), the downward levels
 # 3..min(n-2, 10) run two per launch and the finer ones in the sibling-pair
-# form with 16-byte rows; every level is checked against the oracle's FP64
-# tree at FP32 accuracy.
+# form with 16-byte rows; the section sums and every level are checked
+# against the oracle's FP64 values at FP32 accuracy.
 @pytest.mark.parametrize("n,p,terms,leaf", [(1 << 11, 16, 16, 16),   # depth 3: pairs
                                             (1 << 12, 16, 12, 16),   # depth 4: quads
                                             (1 << 14, 64, 16, 32),   # depth 3, 16-byte rows
