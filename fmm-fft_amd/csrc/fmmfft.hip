@@ -409,6 +409,24 @@ void fft_form(const Geom &g, const FftPass &fp, unsigned nz, hipStream_t st, Sig
     // tiles of all nz signals (a batch is one persistent grid)
     const int64_t ntot = (fp.F * g.P + C - 1) / C * (int64_t)nz;
     const dim3 block(kFftThreads);
+    // complex64 pass 1 over contiguous signals: two-column 16-byte loads
+    // where every such load is 16-byte aligned (adjacent columns, even
+    // strides, an aligned base)
+    if constexpr (FORM == FFT_PASS1 && sizeof(T) == 4 && !TAB && C % 2 == 0) {
+        const bool v2 = fp.in_fs == 1 && fp.in_es % 2 == 0 && fp.in_bs % 2 == 0 && g.M % 2 == 0 &&
+                        reinterpret_cast<uintptr_t>(in.base) % 16 == 0 && (nz == 1 || in.stride % 2 == 0);
+        if (v2 && !force_single) {
+            const unsigned r2 =
+                resident_grid((const void *)k_fft<T, L, FORM, C, true, TAB, true>, kFftThreads, 0, ntot);
+            if ((int64_t)r2 < ntot)
+                hipLaunchKernelGGL((k_fft<T, L, FORM, C, true, TAB, true>), dim3(r2, 1, 1), block, 0, st, g, fp, in,
+                                   out, tw, tlo, thi, (int)nz);
+            else
+                hipLaunchKernelGGL((k_fft<T, L, FORM, C, false, TAB, true>), dim3((unsigned)ntot, 1, 1), block, 0, st,
+                                   g, fp, in, out, tw, tlo, thi, (int)nz);
+            return;
+        }
+    }
     const unsigned res =
         force_single ? (unsigned)ntot : resident_grid((const void *)k_fft<T, L, FORM, C, true, TAB>, kFftThreads, 0, ntot);
     if ((int64_t)res < ntot)

@@ -220,7 +220,12 @@ constexpr int kFftThreads = 512;
 enum FftForm { FFT_SINGLE = 0, FFT_PASS1 = 1, FFT_PASS2 = 2 };
 
 // PERSIST = false: one tile per workgroup (grid = tiles), no prefetch.
-template <typename T, int L, int FORM, int C = fft_cols<T, L>(), bool PERSIST = true, bool TAB = false>
+// V2 (complex64 pass 1, 16-byte aligned input with even column, element and
+// block strides): a lane loads two neighbouring columns of an element as one
+// 16-byte access, halving the pass's load instructions (N = 2^26: pass 1
+// 0.252 -> 0.242 ms; its texture-address unit was the one held up).
+template <typename T, int L, int FORM, int C = fft_cols<T, L>(), bool PERSIST = true, bool TAB = false,
+          bool V2 = false>
 __global__ __launch_bounds__(kFftThreads) void k_fft(Geom g, FftPass fp, SigAddr<const cx<T>> in,
                                              SigAddr<cx<T>> out,
                                              const cx<T> *__restrict__ twL, const c64 *__restrict__ tlo,
@@ -258,12 +263,29 @@ __global__ __launch_bounds__(kFftThreads) void k_fft(Geom g, FftPass fp, SigAddr
     // re/im in separate scalar arrays and unconditional loads (a ragged last
     // tile re-reads its last valid sub-FFT): the prefetch stays in registers.
     T vr[PER], vi[PER];
+    static_assert(!V2 || (FORM == FFT_PASS1 && sizeof(T) == 4 && C % 2 == 0), "two-column loads: complex64 pass 1");
     auto prefetch = [&](int tt) __attribute__((always_inline)) {
         const int sig = tt / ntiles, t = tt - sig * ntiles;
         const cx<T> *in_s = in.template at<TAB>(sig);
         const int fg0 = t * C;
         const int cl = min(C, nf - fg0) - 1;
         const int k2 = fg0 / F, f0 = fg0 - k2 * F, base = k2 * M;
+        if constexpr (V2) {
+            // columns f, f + 1 (f even; a tile holds an even number of
+            // columns, so f <= cl - 1) of element e
+#pragma unroll
+            for (int u = 0; u < PER / 2; ++u) {
+                const int i = threadIdx.x + u * NT;
+                const int f = min(2 * (i % (C / 2)), cl - 1), e = i / (C / 2);
+                const int fo = (f0 >> in_ltw) * fblk_in + ((f0 & ((1 << in_ltw) - 1)) + f) * fstep_in;
+                const float4 v = *reinterpret_cast<const float4 *>(in_s + base + fo + e * in_es);
+                vr[2 * u] = v.x;
+                vi[2 * u] = v.y;
+                vr[2 * u + 1] = v.z;
+                vi[2 * u + 1] = v.w;
+            }
+            return;
+        }
 #pragma unroll
         for (int u = 0; u < PER; ++u) {
             const int i = threadIdx.x + u * NT;
@@ -280,11 +302,21 @@ __global__ __launch_bounds__(kFftThreads) void k_fft(Geom g, FftPass fp, SigAddr
     if (tt < ntot) prefetch(tt);
     while (tt < ntot) {
         __syncthreads();  // the previous tile's outputs have left LDS
+        if constexpr (V2) {
 #pragma unroll
-        for (int u = 0; u < PER; ++u) {
-            const int i = threadIdx.x + u * NT;
-            const int f = erow ? i / L : i % C, e = erow ? i % L : i / C;
-            a[f * ls + lpadp<PSX>(e)] = cx<T>{vr[u], vi[u]};
+            for (int u = 0; u < PER / 2; ++u) {
+                const int i = threadIdx.x + u * NT;
+                const int f = 2 * (i % (C / 2)), e = i / (C / 2);
+                a[f * ls + lpadp<PSX>(e)] = cx<T>{vr[2 * u], vi[2 * u]};
+                a[(f + 1) * ls + lpadp<PSX>(e)] = cx<T>{vr[2 * u + 1], vi[2 * u + 1]};
+            }
+        } else {
+#pragma unroll
+            for (int u = 0; u < PER; ++u) {
+                const int i = threadIdx.x + u * NT;
+                const int f = erow ? i / L : i % C, e = erow ? i % L : i / C;
+                a[f * ls + lpadp<PSX>(e)] = cx<T>{vr[u], vi[u]};
+            }
         }
         __syncthreads();
         const int sig = tt / ntiles, t = tt - sig * ntiles;
