@@ -427,6 +427,23 @@ void fft_form(const Geom &g, const FftPass &fp, unsigned nz, hipStream_t st, Sig
             return;
         }
     }
+    // complex64 last pass into contiguous signals: two-column 16-byte stores
+    // where every such store is 16-byte aligned
+    if constexpr (FORM == FFT_PASS2 && sizeof(T) == 4 && !TAB && C % 2 == 0) {
+        const bool s2 = fp.out_fs == 1 && fp.out_tile == 0 && fp.out_ks % 2 == 0 && !fp.twiddle && g.M % 2 == 0 &&
+                        reinterpret_cast<uintptr_t>(out.base) % 16 == 0 && (nz == 1 || out.stride % 2 == 0);
+        if (s2 && !force_single) {
+            const unsigned r2 =
+                resident_grid((const void *)k_fft<T, L, FORM, C, true, TAB, false, true>, kFftThreads, 0, ntot);
+            if ((int64_t)r2 < ntot)
+                hipLaunchKernelGGL((k_fft<T, L, FORM, C, true, TAB, false, true>), dim3(r2, 1, 1), block, 0, st, g, fp,
+                                   in, out, tw, tlo, thi, (int)nz);
+            else
+                hipLaunchKernelGGL((k_fft<T, L, FORM, C, false, TAB, false, true>), dim3((unsigned)ntot, 1, 1), block, 0,
+                                   st, g, fp, in, out, tw, tlo, thi, (int)nz);
+            return;
+        }
+    }
     const unsigned res =
         force_single ? (unsigned)ntot : resident_grid((const void *)k_fft<T, L, FORM, C, true, TAB>, kFftThreads, 0, ntot);
     if ((int64_t)res < ntot)

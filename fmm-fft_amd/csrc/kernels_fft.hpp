@@ -224,8 +224,10 @@ enum FftForm { FFT_SINGLE = 0, FFT_PASS1 = 1, FFT_PASS2 = 2 };
 // block strides): a lane loads two neighbouring columns of an element as one
 // 16-byte access, halving the pass's load instructions (N = 2^26: pass 1
 // 0.252 -> 0.242 ms; its texture-address unit was the one held up).
+// S2 (complex64 last pass, 16-byte aligned output with even strides): a lane
+// stores two neighbouring sub-FFTs' outputs of one k as one 16-byte access.
 template <typename T, int L, int FORM, int C = fft_cols<T, L>(), bool PERSIST = true, bool TAB = false,
-          bool V2 = false>
+          bool V2 = false, bool S2 = false>
 __global__ __launch_bounds__(kFftThreads) void k_fft(Geom g, FftPass fp, SigAddr<const cx<T>> in,
                                              SigAddr<cx<T>> out,
                                              const cx<T> *__restrict__ twL, const c64 *__restrict__ tlo,
@@ -264,6 +266,7 @@ __global__ __launch_bounds__(kFftThreads) void k_fft(Geom g, FftPass fp, SigAddr
     // tile re-reads its last valid sub-FFT): the prefetch stays in registers.
     T vr[PER], vi[PER];
     static_assert(!V2 || (FORM == FFT_PASS1 && sizeof(T) == 4 && C % 2 == 0), "two-column loads: complex64 pass 1");
+    static_assert(!S2 || (FORM == FFT_PASS2 && sizeof(T) == 4 && C % 2 == 0), "two-column stores: complex64 pass 2");
     auto prefetch = [&](int tt) __attribute__((always_inline)) {
         const int sig = tt / ntiles, t = tt - sig * ntiles;
         const cx<T> *in_s = in.template at<TAB>(sig);
@@ -350,6 +353,23 @@ __global__ __launch_bounds__(kFftThreads) void k_fft(Geom g, FftPass fp, SigAddr
         if constexpr (PERSIST) prefetch(tn < ntot ? tn : tt);
         stockham<T, L, C, 1, NT, PSX>(a, ls, twL);
 
+        if constexpr (S2) {
+            // (no pass-2 twiddle in complex64; out_fs = 1, so sub-FFTs f and
+            // f + 1 are neighbours; a tile holds an even number of them)
+            typedef float f4v __attribute__((ext_vector_type(4)));
+#pragma unroll
+            for (int u = 0; u < PER / 2; ++u) {
+                const int i = threadIdx.x + u * NT;
+                const int f = 2 * (i % (C / 2)), k = i / (C / 2);
+                const cx<T> v0 = a[f * ls + lpadp<PSX>(k)], v1 = a[(f + 1) * ls + lpadp<PSX>(k)];
+                const float sg = fp.conj_out ? -1.f : 1.f;
+                if (f < cv)
+                    __builtin_nontemporal_store(f4v{v0.re, sg * v0.im, v1.re, sg * v1.im},
+                                                reinterpret_cast<f4v *>(out_s + base + (f0 + f) + k * out_ks));
+            }
+            tt = tn;
+            continue;
+        }
         c64 w{1.0, 0.0}, ws{1.0, 0.0};
         if (TWF && fp.twiddle) {
             w = cmul(th0, tl0);
