@@ -1380,7 +1380,9 @@ __host__ __device__ constexpr int tree_mma_rb(int mode) {
 // with S0 = psi_{l-1}[c/2] and S1..S4 = phi_l[c-2], [c-1], [c+2], [c+3],
 // and the 16-byte accesses take a quarter of the address work of the
 // one-row-per-lane form (level 14 at N = 2^26: 107 -> 68 us).
-template <typename T, int TP, int MODE, bool PACK = false, bool SPLIT_M2L = false, bool SIB4 = false>
+// SE (sibling form): rows per lane, 4 (rows a multiple of 64, 16-byte
+// accesses) or 2 (32-row nodes, P = 16: 8-byte accesses, 32 rows per wave).
+template <typename T, int TP, int MODE, bool PACK = false, bool SPLIT_M2L = false, bool SIB4 = false, int SE = 4>
 __global__ __launch_bounds__(256) void k_tree_mma(Geom g, int l, const T *__restrict__ phi_in,
                                                   T *__restrict__ phi_out, const T *__restrict__ psi_in,
                                                   T *__restrict__ psi_out, int64_t tree_stride,
@@ -1453,11 +1455,13 @@ __global__ __launch_bounds__(256) void k_tree_mma(Geom g, int l, const T *__rest
     };
     T *lvl_out = (MODE == 0 ? phi_out : psi_out) + ts + level_off(g, l);
     if constexpr (SIB4) {
-        static_assert(sizeof(T) == 4 && MODE == 1 && !PACK && TP == 16, "sibling form: complex64, downward");
-        const int ng = rows >> 6;  // 64-row groups of a node
+        static_assert(sizeof(T) == 4 && MODE == 1 && !PACK && TP == 16 && (SE == 4 || SE == 2),
+                      "sibling form: complex64, downward");
+        using VT = typename std::conditional<SE == 4, float4, float2>::type;
+        const int ng = rows / (16 * SE);  // row groups of a node
         const int pr = wid / ng, c0 = 2 * pr;
         if (c0 >= nn) return;
-        const int rc = 64 * (wid - pr * ng) + 4 * col;
+        const int rc = 16 * SE * (wid - pr * ng) + SE * col;
         const T *o = opfrag + (size_t)(l - 3) * NOPS * FR;
         // operators of the even node, then of the odd one (source order above)
         constexpr int opi[8] = {OP_SHFTLP, OP_FLIP2P, OP_FLIP2N, OP_FLIP3N, OP_SHFTLN, OP_FLIP3P, OP_FLIP2P, OP_FLIP2N};
@@ -1475,23 +1479,23 @@ __global__ __launch_bounds__(256) void k_tree_mma(Geom g, int l, const T *__rest
         const T *srcs[5] = {psi_in + ts + level_off(g, l - 1) + (int64_t)pr * node_len,
                             ph + (int64_t)((c0 - 2) & m) * node_len, ph + (int64_t)((c0 - 1) & m) * node_len,
                             ph + (int64_t)((c0 + 2) & m) * node_len, ph + (int64_t)((c0 + 3) & m) * node_len};
-        float4 bq[5][KS];
+        VT bq[5][KS];
 #pragma unroll
         for (int t = 0; t < 5; ++t)
 #pragma unroll
             for (int s = 0; s < KS; ++s)
-                bq[t][s] = *reinterpret_cast<const float4 *>(srcs[t] + min(4 * s + kr, terms - 1) * rows + rc);
-        V4 acc[2][4];
+                bq[t][s] = *reinterpret_cast<const VT *>(srcs[t] + min(4 * s + kr, terms - 1) * rows + rc);
+        V4 acc[2][SE];
 #pragma unroll
         for (int h = 0; h < 2; ++h)
 #pragma unroll
-            for (int e = 0; e < 4; ++e) acc[h][e] = V4{0, 0, 0, 0};
+            for (int e = 0; e < SE; ++e) acc[h][e] = V4{0, 0, 0, 0};
 #pragma unroll
         for (int t = 0; t < 4; ++t)
 #pragma unroll
             for (int s = 0; s < KS; ++s)
 #pragma unroll
-                for (int e = 0; e < 4; ++e) {
+                for (int e = 0; e < SE; ++e) {
                     const T be = velem(bq[se[t]][s], e), bo = velem(bq[so[t]][s], e);
                     acc[0][e] = Mfma<T>::mma(a2[t][s], be, acc[0][e]);
                     acc[1][e] = Mfma<T>::mma(a2[4 + t][s], bo, acc[1][e]);
@@ -1507,9 +1511,13 @@ __global__ __launch_bounds__(256) void k_tree_mma(Geom g, int l, const T *__rest
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int k = Mfma<T>::row(kr, r);
-                if (k < terms)
-                    *reinterpret_cast<float4 *>(dst + (int64_t)k * rows) =
-                        make_float4(acc[h][0][r], acc[h][1][r], acc[h][2][r], acc[h][3][r]);
+                if (k < terms) {
+                    if constexpr (SE == 4)
+                        *reinterpret_cast<float4 *>(dst + (int64_t)k * rows) =
+                            make_float4(acc[h][0][r], acc[h][1][r], acc[h][2][r], acc[h][3][r]);
+                    else
+                        *reinterpret_cast<float2 *>(dst + (int64_t)k * rows) = make_float2(acc[h][0][r], acc[h][1][r]);
+                }
             }
         }
         return;
