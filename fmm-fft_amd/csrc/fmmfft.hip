@@ -692,7 +692,8 @@ void s2m_tree_pt(const Geom &g, const Shapes &sh, unsigned nz, hipStream_t st, S
     int lu = g.n - 1 - (sh.s2m_vec ? sh.s2m_levels : 0);
     // complex64, 16 terms, nodes of more than two row blocks: upward levels
     // two per launch, the first of which also reduces the section sums
-    const bool up2 = sizeof(T) == 4 && TP == 16 && !pack && lu - 1 >= 2;
+    // (also for 32-row nodes, P = 16: 256 x 2^20 batch 6.30 -> 6.25 ms)
+    const bool up2 = sizeof(T) == 4 && TP == 16 && (!pack || tree_rows(g) == 32) && lu - 1 >= 2;
     const SumsJob sj{sums_part, (int)sh.s2m_nblk, wstride_c64, sumsec, wstride_c64};
     // downward levels: the split-product M2L instances for FMMFFT_C64_SPLIT
     const bool split = tb.opfrag_lo != nullptr;
