@@ -224,8 +224,13 @@ size_t leaf_h16_lds(const Geom &g, int G) {
     return (2 * G * wstr + (size_t)G * 4 * g.b + 9 * lstr + (size_t)G * 2 * tp + nwv * (size_t)G) * sizeof(float);
 }
 
-// sections per tile of the FP16 leaf kernel (one per wave)
-inline int leaf_h16_G(const Geom &g) { return std::min(g.P, 4); }
+// sections per tile of the FP16 leaf kernel (one per wave).  Leaf 32 takes
+// 8 (64-byte pieces of each x row; one 8-wave workgroup per CU at 158
+// VGPRs): 256 x 2^20 batch at P = 16, leaf 1.94 -> 1.69 ms against 4
+// sections (32-byte pieces, three 4-wave workgroups per CU); 16 sections
+// (whole 128-byte rows) need 1024 threads at 128 VGPRs, spill and took
+// 2.13 ms.  Leaf 64 stays at 4 (8: 0.486 against 0.445 ms at N = 2^26).
+inline int leaf_h16_G(const Geom &g) { return std::min(g.P, g.b == 32 ? 8 : 4); }
 
 size_t leaf_mma_lds(const Geom &g, size_t esize, int G) {
     const int bb = 8 * leaf_ct(g);
