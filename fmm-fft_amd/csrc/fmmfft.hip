@@ -229,7 +229,8 @@ size_t leaf_h16_lds(const Geom &g, int G) {
 // VGPRs): 256 x 2^20 batch at P = 16, leaf 1.94 -> 1.69 ms against 4
 // sections (32-byte pieces, three 4-wave workgroups per CU); 16 sections
 // (whole 128-byte rows) need 1024 threads at 128 VGPRs, spill and took
-// 2.13 ms.  Leaf 64 stays at 4 (8: 0.486 against 0.445 ms at N = 2^26).
+// 2.13 ms.  Leaf 64 (236 VGPRs) stays at 4 (8: 0.438 against 0.432 ms at
+// N = 2^26).
 inline int leaf_h16_G(const Geom &g) { return std::min(g.P, g.b == 32 ? 8 : 4); }
 
 size_t leaf_mma_lds(const Geom &g, size_t esize, int G) {
