@@ -1155,7 +1155,10 @@ int fmmfft_profile_begin(fmmfft_plan plan, int32_t max_calls) {
     plan->prof_ev = new (std::nothrow) hipEvent_t[n];
     if (!plan->prof_ev) return FMMFFT_ENOMEM;
     for (size_t i = 0; i < n; ++i) {
-        hipError_t e = hipEventCreate(&plan->prof_ev[i]);
+        // timing only (fmmfft_profile_end reads timestamps, no data): no
+        // system-scope fence at each record, which would write back and
+        // invalidate the caches between the stages it brackets
+        hipError_t e = hipEventCreateWithFlags(&plan->prof_ev[i], hipEventDisableSystemFence);
         if (e != hipSuccess) {
             for (size_t j = 0; j < i; ++j) (void)hipEventDestroy(plan->prof_ev[j]);
             delete[] plan->prof_ev;
