@@ -61,22 +61,34 @@ def tones_on_device(torch, n, seed):
 
 
 def time_plan(torch, plan, x, out, steps, warmup, profile):
+    """Mean time per execute over `steps` back-to-back executes (one event
+    before and one after them), the per-execute latencies of a second run
+    with an event after every execute (not for profiled runs), and with
+    `profile` the library's per-stage times of the first run."""
     for _ in range(warmup):
         plan.execute(x, out)
     torch.cuda.synchronize()
     if profile:
         plan.profile_begin(steps)
     st = torch.cuda.current_stream()
-    evs = [torch.cuda.Event(enable_timing=True) for _ in range(steps + 1)]
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
-    evs[0].record(st)
-    for i in range(steps):
+    e0.record(st)
+    for _ in range(steps):
         plan.execute(x, out)
-        evs[i + 1].record(st)
+    e1.record(st)
     torch.cuda.synchronize()
-    lat = [evs[i].elapsed_time(evs[i + 1]) for i in range(steps)]
-    total = evs[0].elapsed_time(evs[-1])
+    total = e0.elapsed_time(e1)
     stages = plan.profile_end() if profile else None
+    lat = []
+    if not profile:
+        evs = [torch.cuda.Event(enable_timing=True) for _ in range(steps + 1)]
+        evs[0].record(st)
+        for i in range(steps):
+            plan.execute(x, out)
+            evs[i + 1].record(st)
+        torch.cuda.synchronize()
+        lat = [evs[i].elapsed_time(evs[i + 1]) for i in range(steps)]
     return total / steps, lat, stages
 
 
@@ -414,8 +426,8 @@ def main():
     out32 = torch.empty_like(x32)
     # the transform as a caller runs it (value, latency), then the same
     # workload again with the library's per-stage HIP events on the execute
-    # stream (stage times, roofline): each event record adds a few
-    # microseconds between launches, so the instrumented pass is slower
+    # stream (stage times, roofline): each stage event widens the gap
+    # between two launches, so the instrumented pass is a little slower
     ms32, lat32, _ = time_plan(torch, plan32, x32, out32, args.steps, args.warmup, False)
     ms32_prof, _, stages = time_plan(torch, plan32, x32, out32, args.steps, 1, True)
     gflops32 = 5 * N * math.log2(N) / (ms32 * 1e-3) / 1e9
